@@ -1,0 +1,168 @@
+/*
+ * minipic_c.h -- C ABI of the MI355X-native MiniPIC electrostatic timestep.
+ *
+ * One context owns one mesh on one GPU: all device memory and one HIP stream.
+ * Calls are stream-ordered; a call that returns a host scalar synchronises.
+ * One host thread per context.  Every function returns an int status
+ * (MPIC_OK or a negative MPIC_ERR_*); exceptions never cross this ABI and
+ * mpic_last_error() returns the message of the last failure on a context.
+ *
+ * The reference exposes this path as C++ classes, not an FFI.  Each entry point
+ * below names the reference interface it replaces (file:line in
+ * npe9/miniPIC-crossroads).  The C++ classes of include/minipic/ mirror those
+ * reference classes on top of this ABI.
+ *
+ * Array layouts (host and device):
+ *   particle positions / reference positions / velocities: component planes,
+ *     element [d*stride + p] for component d of particle p;
+ *   per-element arrays: element-major ([elem][...]).
+ */
+#ifndef MINIPIC_C_H_
+#define MINIPIC_C_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPIC_OK 0
+#define MPIC_ERR_ARG (-1)           /* bad argument or shape */
+#define MPIC_ERR_CAPACITY (-2)      /* particle capacity exceeded (particle_fill.hpp:61-64) */
+#define MPIC_ERR_NOT_CONVERGED (-3) /* solver hit max_iter (CG_solve.hpp:73,113) */
+#define MPIC_ERR_HIP (-4)           /* a HIP runtime call failed */
+#define MPIC_ERR_WALK_CAP (-5)      /* a particle exceeded the per-step crossing cap */
+#define MPIC_ERR_STATE (-6)         /* call out of order (e.g. no particles) */
+#define MPIC_ERR_NO_DEVICE (-7)     /* no usable gfx950 device */
+
+#define MPIC_FORM_FP64 0  /* plain form: FP64 throughout, the drop-in */
+#define MPIC_FORM_MIXED 1 /* precision form: FP32 particles, FP32 inner CG in FP64 refinement */
+
+#define MPIC_MAX_SPECIES 16
+
+typedef struct mpic_ctx mpic_ctx;   /* device state of one problem */
+typedef struct mpic_mesh mpic_mesh; /* host-side mesh + discretisation */
+
+/* Host arrays describing a P1 tet mesh and its discretisation.  Borrowed:
+ * mpic_create copies them to the device during the call. */
+typedef struct {
+  int32_t num_nodes;
+  int32_t num_elems;
+  int64_t nnz;
+  const double* nodes;          /* [num_nodes][3]          Mesh::nodes, mesh.hpp:111 */
+  const int32_t* elem_nodes;    /* [num_elems][4]          Mesh::elem_to_node_lids, mesh.hpp:136 */
+  const int32_t* elem_nbrs;     /* [num_elems][4], -1 wall Mesh::elem_face_to_elem_gids, mesh.hpp:144 */
+  const double* normals;        /* [num_elems][4][3]       Mesh::normals, mesh.hpp:112 */
+  const double* inv_jacobian;   /* [num_elems][3][3]       Mesh::inverse_jacobian(e,0), mesh.hpp:153 */
+  const double* det_jacobian;   /* [num_elems]             Mesh::determinate_jacobian, mesh.hpp:154 */
+  const double* grad_basis;     /* [num_elems][4][3]       PIC::grad_phi_basis_values_(e,i,*), PIC.cpp:213-241 */
+  const double* wbv;            /* [num_elems][4][4]       ES::weighted_basis_values_, ES.cpp:193-195 */
+  const uint8_t* boundary;      /* [num_nodes] 1 = Dirichlet node, Mesh::boundary_node_map, mesh.cpp:408-412 */
+  const int32_t* row_ptr;       /* [num_nodes+1] stiffness CSR (ES::matrix_, ES.cpp:83-205) */
+  const int32_t* col_idx;       /* [nnz], ascending within a row */
+  const double* values;         /* [nnz] */
+  const double* background;     /* [num_nodes] constant rhs term (ES::set_analytic_RHS, ES.cpp:235-257) or NULL */
+} mpic_mesh_desc;
+
+typedef struct {
+  int form;                              /* MPIC_FORM_* */
+  double cg_tol;                         /* relative residual tolerance (ES::solve(tol), ES.cpp:216) */
+  int cg_max_iter;                       /* CG_solve.hpp:73 uses 1000 */
+  int kill_on_wall;                      /* DataWarehouse::set_kill_particles_on_wall_impact, data_warehouse.hpp:76 */
+  int sort_period;                       /* counting sort by element every N time steps (0 = never) */
+  int num_species;                       /* ParticleTypeList, particle_type.hpp:58-86 */
+  double species_q[MPIC_MAX_SPECIES];
+  double species_m[MPIC_MAX_SPECIES];
+} mpic_config;
+
+/* Host particle SoA (planes as described above).  Borrowed for the call. */
+typedef struct {
+  double* x;       /* [3][stride]  ParticleList::x      particle_list.hpp:71 */
+  double* x_ref;   /* [3][stride]  ParticleList::x_ref  particle_list.hpp:73 */
+  double* v;       /* [3][stride]  ParticleList::v      particle_list.hpp:75 */
+  int32_t* elem;   /* [stride]     ParticleList::ielement, -1 = dead  particle_list.hpp:69 */
+  double* weight;  /* [stride]     ParticleList::weight particle_list.hpp:85 */
+  int32_t* type;   /* [stride]     ParticleList::type   particle_list.hpp:87 */
+  uint32_t* id;    /* [stride]     identity key (fill order), kept through sorts */
+  int64_t stride;
+} mpic_soa_host;
+
+typedef struct {
+  int64_t crossings;   /* element-face crossings in the tet walk */
+  int64_t killed;      /* particles removed at a wall (kill_on_wall) */
+  int64_t capped;      /* particles that hit the crossing cap (reference: unbounded loop) */
+  int64_t nonpositive; /* sub-steps with s <= 0 (the reference asserts s > 0, particle_move.hpp:223) */
+  int64_t reflections; /* specular wall reflections (particle_move.hpp:213-220) */
+} mpic_move_stats;
+
+typedef struct {
+  int cg_iters;
+  double cg_relres;
+  double kinetic_energy;   /* PIC.cpp:255-264 */
+  mpic_move_stats move;
+  int64_t num_particles;   /* after the step */
+  int charge_scale_bits;   /* fixed-point scale used for the node charge sum */
+  double ms_deposit, ms_solve, ms_push, ms_sort, ms_ke;  /* GPU time of the phases */
+} mpic_step_stats;
+
+/* ---- configuration -------------------------------------------------------- */
+/* Defaults: FP64 form, CG tol 1e-10, max_iter 1000, reflect on walls, sort every
+ * 4 steps, species e-(-1,1) e+(1,1) H+(1,1837) as particle_type.hpp:65-74. */
+void mpic_config_default(mpic_config* cfg);
+const char* mpic_last_error(const mpic_ctx* ctx); /* ctx may be NULL: last create/mesh error */
+
+/* ---- host mesh (Mesh::Mesh mesh.cpp:78-714, ES::ES ES.cpp:47-213, PIC::PIC PIC.cpp:203-242) ---- */
+/* Brick of nx*ny*nz hexes on [lo,hi], split = 5 (reference, mesh.cpp:83) or 6
+ * (Kuhn); interior nodes jittered by +-jitter*h (0 = none). */
+int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
+                    uint64_t jitter_seed, mpic_mesh** out);
+/* Uniform background charge density f as an rhs load vector (ES::set_analytic_RHS with a constant). */
+int mpic_mesh_set_background(mpic_mesh* mesh, double f);
+/* Borrowed views of the mesh arrays (valid until mpic_mesh_free). */
+int mpic_mesh_describe(const mpic_mesh* mesh, mpic_mesh_desc* out);
+void mpic_mesh_free(mpic_mesh* mesh);
+
+/* ---- context -------------------------------------------------------------- */
+/* DataWarehouse::create_mesh/create_ES + PIC::PIC upload (data_warehouse.cpp:50-73). */
+int mpic_create(const mpic_mesh_desc* mesh, const mpic_config* cfg, mpic_ctx** out);
+void mpic_destroy(mpic_ctx* ctx);
+int mpic_set_form(mpic_ctx* ctx, int form);
+
+/* ---- particles (ParticleList, particle_list.hpp:54-320) -------------------- */
+int mpic_particles_reserve(mpic_ctx* ctx, int64_t capacity);          /* DataWarehouse::create_particle_list */
+int mpic_particles_upload(mpic_ctx* ctx, const mpic_soa_host* h, int64_t n);
+int mpic_particles_download(mpic_ctx* ctx, mpic_soa_host* h, int64_t* n);
+int mpic_num_particles(mpic_ctx* ctx, int64_t* n);                    /* ParticleList::num_particles */
+/* ParticleFill(data, nppe, type, density, vel, temp), particle_fill.hpp:59-164, on
+ * the device; appends.  seed_offset 0 = the reference's seeds; pert_amp scales
+ * weights by 1 + pert_amp*sin(2 pi (x - lo_x)/L_x). */
+int mpic_fill(mpic_ctx* ctx, int nppe, int type, double density, const double* vel, double temp,
+              uint64_t seed_offset, double pert_amp);
+
+/* ---- phases of PIC::time_step (PIC.cpp:245-267) ----------------------------- */
+int mpic_weight_charge(mpic_ctx* ctx);                                 /* PIC::weight_charge, PIC.cpp:139-160 */
+int mpic_solve(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres); /* ES::solve, ES.cpp:216-231 + CG_solve.hpp:54-114 */
+int mpic_weight_efield(mpic_ctx* ctx);                                 /* PIC::weight_Efield, PIC.cpp:162-189 */
+int mpic_accel(mpic_ctx* ctx, double dt, double scale);                /* ParticleMove AccelParticlesTag, particle_move.hpp:364-392 */
+int mpic_move(mpic_ctx* ctx, double dt, mpic_move_stats* stats);       /* MoveParticle::execute, particle_move.hpp:72-124 */
+int mpic_compute_ke(mpic_ctx* ctx, double* ke);                        /* PICComputeKETag, PIC.cpp:191-200 */
+int mpic_sort_by_element(mpic_ctx* ctx, int32_t* start_opt);           /* ParticleList::sort + compute_start_particles_by_element, particle_list.cpp:90-146 */
+int mpic_time_step(mpic_ctx* ctx, double dt, mpic_step_stats* stats);  /* PIC::time_step, fused */
+
+/* ---- field access ----------------------------------------------------------- */
+int mpic_get_rhs(mpic_ctx* ctx, double* rhs);        /* ES::rhs(lid), ES.hpp:63 */
+int mpic_get_phi(mpic_ctx* ctx, double* phi);        /* ES::phi(lid), ES.hpp:58 */
+int mpic_set_phi(mpic_ctx* ctx, const double* phi);
+int mpic_get_efield(mpic_ctx* ctx, double* E, int64_t stride); /* ParticleList::E after weight_efield, [3][stride] */
+int mpic_charge_scale(mpic_ctx* ctx, int* bits);
+
+/* ---- instrumentation (PIC::output_timers, PIC.hpp:85-88) --------------------- */
+/* Average device duration (ms) of the last launch of the named kernel family:
+ * 0 push, 1 deposit, 2 node gather, 3 CG iteration, 4 sort scatter. */
+int mpic_kernel_time(mpic_ctx* ctx, int which, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MINIPIC_C_H_ */

@@ -1,0 +1,795 @@
+// C ABI of the MI355X MiniPIC library (include/minipic_c.h): context, device
+// memory, and the host-side ordering of the timestep phases (PIC::time_step,
+// PIC.cpp:245-267).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpic_internal.hpp"
+
+namespace mpic {
+std::string& global_error() {
+  static thread_local std::string e;
+  return e;
+}
+}  // namespace mpic
+
+using namespace mpic;
+
+namespace {
+
+enum Timer { T_DEPOSIT, T_SOLVE, T_PUSH, T_SORT, T_KE, T_PUSH_KERNEL, T_COUNT };
+
+}  // namespace
+
+struct mpic_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mpic_config cfg{};
+  std::string err;
+  std::vector<void*> allocs;  // mesh-lifetime device buffers
+
+  DevMesh m{};
+  std::vector<double> background;  // host copy
+  double bg_abs_sum = 0;
+  fx128* bgfix = nullptr;
+  double* species = nullptr;  // [48]
+
+  // particles (two SoA sets for the sort)
+  int64_t cap = 0, n = 0;
+  DevParticles P[2]{};
+  int cur = 0;
+  void* pmem[2] = {nullptr, nullptr};
+  double* E = nullptr;   // transient [3][cap]
+  double* vr = nullptr;  // transient [3][cap]
+  PushPartial* partial = nullptr;
+  PushPartial* finish = nullptr;   // device [1]
+  PushPartial* finish_h = nullptr; // pinned host [1]
+  unsigned long long* bound_d = nullptr;  // [2]
+  unsigned long long* bound_h = nullptr;  // pinned [2]
+
+  // fields
+  double* phi = nullptr;
+  double* rhs = nullptr;
+  fx128* acc = nullptr;  // [ne*4]
+  CGBuffers cg{};
+  CGState* cg_h = nullptr;  // pinned
+  int last_cg_iters = 0;
+
+  SortBuffers sb{};
+  int32_t* start_h = nullptr;  // pinned [1]
+
+  // state
+  bool acc_deposited = false;
+  int S = 100;
+  double ke_max_prev = 0;
+  int64_t steps = 0;
+  hipEvent_t ev[2 * T_COUNT] = {};
+  double ms[T_COUNT] = {};
+};
+
+namespace {
+
+int fail(mpic_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    global_error() = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                           \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return fail(c, MPIC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(mpic_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(c, hipMalloc(&q, count * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return MPIC_OK;
+}
+
+template <class T>
+int dupload(mpic_ctx* c, T** p, const T* host, size_t count) {
+  int rc = dalloc(c, p, count);
+  if (rc) return rc;
+  if (count) HIPCHK(c, hipMemcpy(*p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return MPIC_OK;
+}
+
+void tstart(mpic_ctx* c, int t) { (void)hipEventRecord(c->ev[2 * t], c->stream); }
+void tstop(mpic_ctx* c, int t) { (void)hipEventRecord(c->ev[2 * t + 1], c->stream); }
+double telapsed(mpic_ctx* c, int t) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev[2 * t], c->ev[2 * t + 1]) != hipSuccess) return 0;
+  return ms;
+}
+
+PushArgs push_args(mpic_ctx* c) {
+  PushArgs a{};
+  a.m = c->m;
+  a.P = c->P[c->cur];
+  a.n = c->n;
+  a.phi = c->phi;
+  a.dt = 0;
+  a.scale = 1.0;
+  a.kill = c->cfg.kill_on_wall;
+  a.S_charge = c->S;
+  a.S_ke = 100;
+  a.acc_tet = c->acc;
+  a.partial = c->partial;
+  a.E = c->E;
+  a.vr = c->vr;
+  a.species = c->species;
+  return a;
+}
+
+int nblocks(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+// Reduce the per-workgroup partials of the last particle launch; synchronises.
+int finish_partials(mpic_ctx* c, int parts, PushPartial* out) {
+  HIPCHK(c, launch_push_finish(c->partial, parts, c->finish, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->finish_h, c->finish, sizeof(PushPartial), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = *c->finish_h;
+  return MPIC_OK;
+}
+
+// Charge fixed-point scale from the particles: bound = sum|bg| + n_live * max|w q|.
+int update_charge_scale(mpic_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(c->bound_d, 0, 16, c->stream));
+  HIPCHK(c, launch_charge_bound(c->P[c->cur], c->n, c->species, c->bound_d, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->bound_h, c->bound_d, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double mx;
+  std::memcpy(&mx, &c->bound_h[0], 8);
+  const double live = (double)c->bound_h[1];
+  const int S = scale_for_bound(c->bg_abs_sum + live * mx);
+  if (S != c->S || !c->bgfix) {
+    c->S = S;
+    std::vector<fx128> h(c->m.nn, fx128{0, 0});
+    if (!c->background.empty())
+      for (int32_t i = 0; i < c->m.nn; ++i) h[i] = to_fixed(c->background[i], S);
+    if (!c->bgfix) {
+      int rc = dalloc(c, &c->bgfix, c->m.nn);
+      if (rc) return rc;
+      c->m.bgfix = c->bgfix;
+    }
+    HIPCHK(c, hipMemcpy(c->bgfix, h.data(), h.size() * sizeof(fx128), hipMemcpyHostToDevice));
+  }
+  return MPIC_OK;
+}
+
+// Particle positions changed outside the fused path: drop a stale deposit.
+int invalidate_deposit(mpic_ctx* c) {
+  if (c->acc_deposited) {
+    HIPCHK(c, launch_zero(c->acc, (size_t)c->m.ne * 4 * sizeof(fx128), c->stream));
+    c->acc_deposited = false;
+  }
+  return MPIC_OK;
+}
+
+int ensure_transients(mpic_ctx* c) {
+  if (!c->E) {
+    HIPCHK(c, hipMalloc(&c->E, (size_t)3 * c->cap * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->vr, (size_t)3 * c->cap * sizeof(double)));
+  }
+  return MPIC_OK;
+}
+
+int free_particles(mpic_ctx* c) {
+  for (int k = 0; k < 2; ++k)
+    if (c->pmem[k]) {
+      (void)hipFree(c->pmem[k]);
+      c->pmem[k] = nullptr;
+    }
+  if (c->E) (void)hipFree(c->E);
+  if (c->vr) (void)hipFree(c->vr);
+  if (c->partial) (void)hipFree(c->partial);
+  c->E = c->vr = nullptr;
+  c->partial = nullptr;
+  c->cap = c->n = 0;
+  return MPIC_OK;
+}
+
+int do_sort(mpic_ctx* c, int32_t* start_opt) {
+  const int src = c->cur, dst = 1 - c->cur;
+  tstart(c, T_SORT);
+  HIPCHK(c, launch_sort(c->m, c->P[src], c->P[dst], c->n, c->sb, c->stream));
+  tstop(c, T_SORT);
+  HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + c->m.ne, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  if (start_opt)
+    HIPCHK(c, hipMemcpyAsync(start_opt, c->sb.start, ((size_t)c->m.ne + 1) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cur = dst;
+  c->n = *c->start_h;
+  c->ms[T_SORT] = telapsed(c, T_SORT);
+  return MPIC_OK;
+}
+
+int do_weight_charge(mpic_ctx* c) {
+  if (!c->acc_deposited)
+    HIPCHK(c, launch_deposit(c->m, c->P[c->cur], c->n, c->species, c->S, c->acc, c->stream));
+  HIPCHK(c, launch_node_charge(c->m, c->acc, c->S, c->rhs, c->stream));
+  c->acc_deposited = false;
+  return MPIC_OK;
+}
+
+int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
+  c->cg.x = c->phi;
+  c->cg.b = c->rhs;
+  HIPCHK(c, launch_cg_init(c->m, c->cg, tol, max_iter, c->stream));
+  int it = 0;
+  int batch = std::max(8, std::min(max_iter, c->last_cg_iters));
+  while (true) {
+    const int upto = std::min(max_iter, it + batch);
+    for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, c->cg, it, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cg_h, c->cg.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->cg_h->done || it >= max_iter) break;
+    batch = 8;
+  }
+  const CGState& s = *c->cg_h;
+  c->last_cg_iters = s.iters;
+  if (iters) *iters = s.iters;
+  if (relres) *relres = s.rho0 > 0 ? std::sqrt(s.rho / s.rho0) : 0.0;
+  if (!s.converged) return fail(c, MPIC_ERR_NOT_CONVERGED, "CG did not converge within max_iter");
+  return MPIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mpic_config_default(mpic_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->form = MPIC_FORM_FP64;
+  cfg->cg_tol = 1e-10;
+  cfg->cg_max_iter = 1000;
+  cfg->kill_on_wall = 0;
+  cfg->sort_period = 4;
+  cfg->num_species = 3;
+  for (int t = 0; t < MPIC_MAX_SPECIES; ++t) cfg->species_m[t] = 1.0;
+  cfg->species_q[0] = -1.0;  // e-  (particle_type.hpp:69-71)
+  cfg->species_q[1] = 1.0;   // e+
+  cfg->species_q[2] = 1.0;   // H+
+  cfg->species_m[2] = 1837.0;
+}
+
+const char* mpic_last_error(const mpic_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error().c_str(); }
+
+int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out) {
+  if (!d || !out || d->num_nodes <= 0 || d->num_elems <= 0 || !d->nodes || !d->elem_nodes || !d->elem_nbrs ||
+      !d->normals || !d->inv_jacobian || !d->det_jacobian || !d->grad_basis || !d->wbv || !d->boundary ||
+      !d->row_ptr || !d->col_idx || !d->values)
+    return fail(nullptr, MPIC_ERR_ARG, "mpic_create: incomplete mesh description");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, MPIC_ERR_NO_DEVICE, "mpic_create: no HIP device");
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, MPIC_ERR_NO_DEVICE, std::string("mpic_create: device is not gfx950: ") + prop.gcnArchName);
+
+  mpic_ctx* c = new mpic_ctx();
+  c->device = dev;
+  if (cfg)
+    c->cfg = *cfg;
+  else
+    mpic_config_default(&c->cfg);
+  auto bail = [&](int rc) {
+    global_error() = c->err;
+    mpic_destroy(c);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(MPIC_ERR_HIP);
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(MPIC_ERR_HIP);
+
+  const int32_t nn = d->num_nodes, ne = d->num_elems;
+  c->m.nn = nn;
+  c->m.ne = ne;
+  // Per-element records.
+  std::vector<double> geo((size_t)ne * 12), sixodet(ne);
+  std::vector<int32_t> conn((size_t)ne * 4), nbrs((size_t)ne * 4);
+  for (int64_t e = 0; e < ne; ++e) {
+    std::memcpy(&geo[e * 12], d->inv_jacobian + e * 9, 9 * sizeof(double));
+    const int32_t n0 = d->elem_nodes[e * 4];
+    if (n0 < 0 || n0 >= nn) return bail(fail(c, MPIC_ERR_ARG, "mpic_create: element node out of range"));
+    std::memcpy(&geo[e * 12 + 9], d->nodes + (size_t)n0 * 3, 3 * sizeof(double));
+    sixodet[e] = 6 / d->det_jacobian[e];  // the (6/ detJ) of PIC.cpp:125
+    for (int k = 0; k < 4; ++k) {
+      const int32_t v = d->elem_nodes[e * 4 + k], b = d->elem_nbrs[e * 4 + k];
+      if (v < 0 || v >= nn || b < -1 || b >= ne)
+        return bail(fail(c, MPIC_ERR_ARG, "mpic_create: connectivity out of range"));
+      conn[e * 4 + k] = v;
+      nbrs[e * 4 + k] = b;
+    }
+  }
+  // Node -> element slot adjacency (slot = e*4 + k), element order.
+  std::vector<int32_t> adj_ptr(nn + 1, 0), adj((size_t)ne * 4);
+  for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj_ptr[conn[s] + 1]++;
+  for (int32_t i = 0; i < nn; ++i) adj_ptr[i + 1] += adj_ptr[i];
+  {
+    std::vector<int32_t> pos(adj_ptr.begin(), adj_ptr.end() - 1);
+    for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj[pos[conn[s]]++] = (int32_t)s;
+  }
+  // Sliced ELL from CSR.
+  const int32_t nsl = (nn + ELL_SLICE - 1) / ELL_SLICE;
+  std::vector<int64_t> ell_off(nsl + 1, 0);
+  std::vector<int32_t> ell_len(nsl, 0);
+  for (int32_t s = 0; s < nsl; ++s) {
+    int32_t L = 0;
+    for (int l = 0; l < ELL_SLICE; ++l) {
+      const int64_t r = (int64_t)s * ELL_SLICE + l;
+      if (r < nn) L = std::max(L, d->row_ptr[r + 1] - d->row_ptr[r]);
+    }
+    ell_len[s] = L;
+    ell_off[s + 1] = ell_off[s] + (int64_t)L * ELL_SLICE;
+  }
+  std::vector<int32_t> ell_col(ell_off[nsl]);
+  std::vector<double> ell_val(ell_off[nsl]);
+  for (int32_t s = 0; s < nsl; ++s)
+    for (int l = 0; l < ELL_SLICE; ++l) {
+      const int64_t r = (int64_t)s * ELL_SLICE + l;
+      for (int k = 0; k < ell_len[s]; ++k) {
+        const int64_t at = ell_off[s] + (int64_t)k * ELL_SLICE + l;
+        if (r < nn && k < d->row_ptr[r + 1] - d->row_ptr[r]) {
+          const int32_t cc = d->col_idx[d->row_ptr[r] + k];
+          if (cc < 0 || cc >= nn) return bail(fail(c, MPIC_ERR_ARG, "mpic_create: CSR column out of range"));
+          ell_col[at] = cc;
+          ell_val[at] = d->values[d->row_ptr[r] + k];
+        } else {
+          ell_col[at] = r < nn ? (int32_t)r : 0;
+          ell_val[at] = 0.0;
+        }
+      }
+    }
+  double* dgeo;
+  double* dgrad;
+  double* dnormals;
+  double* dsix;
+  double* ddet;
+  double* dwbv;
+  double* dnodes;
+  int32_t *dconn, *dnbrs, *dadjp, *dadj, *dlen, *dcol;
+  int64_t* doff;
+  double* dval;
+  uint8_t* dbnd;
+  int rc;
+#define UP(ptr, vec) \
+  if ((rc = dupload(c, &ptr, vec, count_))) return bail(rc);
+  size_t count_;
+  count_ = geo.size(); UP(dgeo, geo.data());
+  count_ = (size_t)ne * 12; UP(dgrad, d->grad_basis);
+  count_ = (size_t)ne * 12; UP(dnormals, d->normals);
+  count_ = (size_t)ne; UP(dsix, sixodet.data());
+  count_ = (size_t)ne; UP(ddet, d->det_jacobian);
+  count_ = (size_t)ne * 16; UP(dwbv, d->wbv);
+  count_ = (size_t)nn * 3; UP(dnodes, d->nodes);
+  count_ = conn.size(); UP(dconn, conn.data());
+  count_ = nbrs.size(); UP(dnbrs, nbrs.data());
+  count_ = adj_ptr.size(); UP(dadjp, adj_ptr.data());
+  count_ = adj.size(); UP(dadj, adj.data());
+  count_ = ell_len.size(); UP(dlen, ell_len.data());
+  count_ = ell_off.size(); UP(doff, ell_off.data());
+  count_ = ell_col.size(); UP(dcol, ell_col.data());
+  count_ = ell_val.size(); UP(dval, ell_val.data());
+  count_ = (size_t)nn; UP(dbnd, d->boundary);
+#undef UP
+  c->m.geo = dgeo;
+  c->m.grad = dgrad;
+  c->m.normals = dnormals;
+  c->m.sixodet = dsix;
+  c->m.detj = ddet;
+  c->m.wbv = dwbv;
+  c->m.nodes = dnodes;
+  c->m.conn = reinterpret_cast<const int4*>(dconn);
+  c->m.nbrs = reinterpret_cast<const int4*>(dnbrs);
+  c->m.adj_ptr = dadjp;
+  c->m.adj = dadj;
+  c->m.nslices = nsl;
+  c->m.ell_len = dlen;
+  c->m.ell_off = doff;
+  c->m.ell_col = dcol;
+  c->m.ell_val = dval;
+  c->m.bnd = dbnd;
+  c->m.bgfix = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    c->m.lo[a] = 1e300;
+    c->m.hi[a] = -1e300;
+  }
+  for (int32_t i = 0; i < nn; ++i)
+    for (int a = 0; a < 3; ++a) {
+      c->m.lo[a] = std::min(c->m.lo[a], d->nodes[(size_t)i * 3 + a]);
+      c->m.hi[a] = std::max(c->m.hi[a], d->nodes[(size_t)i * 3 + a]);
+    }
+  if (d->background) {
+    c->background.assign(d->background, d->background + nn);
+    for (int32_t i = 0; i < nn; ++i) c->bg_abs_sum += std::fabs(c->background[i]);
+  }
+  // Species table [q | m | q/m].
+  double sp[48];
+  for (int t = 0; t < MPIC_MAX_SPECIES; ++t) {
+    sp[t] = c->cfg.species_q[t];
+    sp[16 + t] = c->cfg.species_m[t];
+    sp[32 + t] = c->cfg.species_m[t] != 0 ? c->cfg.species_q[t] / c->cfg.species_m[t] : 0.0;  // particle_type.hpp:51
+  }
+  if ((rc = dupload(c, &c->species, sp, 48))) return bail(rc);
+  // Fields, CG and sort buffers.
+  const int nchunks = (nn + BLOCK - 1) / BLOCK;
+  if ((rc = dalloc(c, &c->phi, nn)) || (rc = dalloc(c, &c->rhs, nn)) || (rc = dalloc(c, &c->acc, (size_t)ne * 4)) ||
+      (rc = dalloc(c, &c->cg.r, nn)) || (rc = dalloc(c, &c->cg.p0, nn)) || (rc = dalloc(c, &c->cg.p1, nn)) ||
+      (rc = dalloc(c, &c->cg.q, nn)) || (rc = dalloc(c, &c->cg.part_bb, nchunks)) ||
+      (rc = dalloc(c, &c->cg.part_rr, nchunks)) || (rc = dalloc(c, &c->cg.part_pq, nchunks)) ||
+      (rc = dalloc(c, &c->cg.st, 1)) || (rc = dalloc(c, &c->sb.count, (size_t)ne + 2)) ||
+      (rc = dalloc(c, &c->sb.start, (size_t)ne + 2)) || (rc = dalloc(c, &c->sb.cursor, (size_t)ne + 2)) ||
+      (rc = dalloc(c, &c->sb.blocksum, (size_t)ne / 2048 + 64)) || (rc = dalloc(c, &c->finish, 1)) ||
+      (rc = dalloc(c, &c->bound_d, 2)))
+    return bail(rc);
+  c->cg.nchunks = nchunks;
+  if (hipHostMalloc((void**)&c->cg_h, sizeof(CGState)) != hipSuccess ||
+      hipHostMalloc((void**)&c->finish_h, sizeof(PushPartial)) != hipSuccess ||
+      hipHostMalloc((void**)&c->bound_h, 16) != hipSuccess ||
+      hipHostMalloc((void**)&c->start_h, sizeof(int32_t)) != hipSuccess)
+    return bail(fail(c, MPIC_ERR_HIP, "pinned host allocation failed"));
+  if (hipMemset(c->phi, 0, (size_t)nn * 8) != hipSuccess || hipMemset(c->rhs, 0, (size_t)nn * 8) != hipSuccess ||
+      hipMemset(c->acc, 0, (size_t)ne * 4 * sizeof(fx128)) != hipSuccess ||
+      hipMemset(c->cg.st, 0, sizeof(CGState)) != hipSuccess)
+    return bail(fail(c, MPIC_ERR_HIP, "hipMemset failed"));
+  if ((rc = update_charge_scale(c))) return bail(rc);
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MPIC_ERR_HIP, "setup synchronisation failed"));
+  *out = c;
+  return MPIC_OK;
+}
+
+void mpic_destroy(mpic_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_particles(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->cg_h) (void)hipHostFree(c->cg_h);
+  if (c->finish_h) (void)hipHostFree(c->finish_h);
+  if (c->bound_h) (void)hipHostFree(c->bound_h);
+  if (c->start_h) (void)hipHostFree(c->start_h);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mpic_set_form(mpic_ctx* c, int form) {
+  if (!c) return MPIC_ERR_ARG;
+  if (form != MPIC_FORM_FP64) return fail(c, MPIC_ERR_ARG, "mpic_set_form: form not available");
+  c->cfg.form = form;
+  return MPIC_OK;
+}
+
+int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
+  if (!c || cap <= 0 || cap > ((int64_t)1 << 31) - 1) return fail(c, MPIC_ERR_ARG, "mpic_particles_reserve: bad capacity");
+  free_particles(c);
+  const size_t per = 10 * sizeof(double) + sizeof(int32_t) + sizeof(uint32_t) + 1;
+  for (int k = 0; k < 2; ++k) {
+    char* base = nullptr;
+    HIPCHK(c, hipMalloc((void**)&base, per * (size_t)cap + 1024));
+    c->pmem[k] = base;
+    DevParticles& P = c->P[k];
+    P.cap = cap;
+    P.x = (double*)base;
+    P.xr = P.x + 3 * cap;
+    P.v = P.xr + 3 * cap;
+    P.w = P.v + 3 * cap;
+    P.elem = (int32_t*)(P.w + cap);
+    P.id = (uint32_t*)(P.elem + cap);
+    P.type = (uint8_t*)(P.id + cap);
+  }
+  HIPCHK(c, hipMalloc((void**)&c->partial, sizeof(PushPartial) * (size_t)nblocks(cap)));
+  c->cap = cap;
+  c->n = 0;
+  c->cur = 0;
+  c->acc_deposited = false;
+  c->ke_max_prev = 0;
+  HIPCHK(c, hipMemset(c->acc, 0, (size_t)c->m.ne * 4 * sizeof(fx128)));
+  return MPIC_OK;
+}
+
+int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
+  if (!c || !h || n < 0 || !h->x || !h->x_ref || !h->v || !h->elem || !h->weight || !h->type || h->stride < n)
+    return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: bad arguments");
+  if (n > c->cap) return fail(c, MPIC_ERR_CAPACITY, "mpic_particles_upload: capacity exceeded");
+  std::vector<uint8_t> t(n);
+  std::vector<uint32_t> ids(n);
+  for (int64_t p = 0; p < n; ++p) {
+    if (h->type[p] < 0 || h->type[p] >= c->cfg.num_species)
+      return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: species out of range");
+    if (h->elem[p] < -1 || h->elem[p] >= c->m.ne) return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: element out of range");
+    t[p] = (uint8_t)h->type[p];
+    ids[p] = h->id ? h->id[p] : (uint32_t)p;
+  }
+  DevParticles& P = c->P[c->cur];
+  for (int d = 0; d < 3; ++d) {
+    HIPCHK(c, hipMemcpy(P.x + d * P.cap, h->x + d * h->stride, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(P.xr + d * P.cap, h->x_ref + d * h->stride, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(P.v + d * P.cap, h->v + d * h->stride, n * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(P.w, h->weight, n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(P.elem, h->elem, n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(P.type, t.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(P.id, ids.data(), n * 4, hipMemcpyHostToDevice));
+  c->n = n;
+  c->ke_max_prev = 0;
+  int rc = invalidate_deposit(c);
+  if (rc) return rc;
+  return update_charge_scale(c);
+}
+
+int mpic_particles_download(mpic_ctx* c, mpic_soa_host* h, int64_t* n) {
+  if (!c || !h || h->stride < c->n) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: bad arguments");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevParticles& P = c->P[c->cur];
+  const int64_t cnt = c->n;
+  for (int d = 0; d < 3; ++d) {
+    if (h->x) HIPCHK(c, hipMemcpy(h->x + d * h->stride, P.x + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
+    if (h->x_ref) HIPCHK(c, hipMemcpy(h->x_ref + d * h->stride, P.xr + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
+    if (h->v) HIPCHK(c, hipMemcpy(h->v + d * h->stride, P.v + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
+  }
+  if (h->weight) HIPCHK(c, hipMemcpy(h->weight, P.w, cnt * 8, hipMemcpyDeviceToHost));
+  if (h->elem) HIPCHK(c, hipMemcpy(h->elem, P.elem, cnt * 4, hipMemcpyDeviceToHost));
+  if (h->id) HIPCHK(c, hipMemcpy(h->id, P.id, cnt * 4, hipMemcpyDeviceToHost));
+  if (h->type) {
+    std::vector<uint8_t> t(cnt);
+    HIPCHK(c, hipMemcpy(t.data(), P.type, cnt, hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < cnt; ++p) h->type[p] = t[p];
+  }
+  if (n) *n = cnt;
+  return MPIC_OK;
+}
+
+int mpic_num_particles(mpic_ctx* c, int64_t* n) {
+  if (!c || !n) return MPIC_ERR_ARG;
+  *n = c->n;
+  return MPIC_OK;
+}
+
+int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel, double temp, uint64_t seed_offset,
+              double pert_amp) {
+  if (!c || nppe <= 0 || type < 0 || type >= c->cfg.num_species) return fail(c, MPIC_ERR_ARG, "mpic_fill: bad arguments");
+  const int64_t add = (int64_t)c->m.ne * nppe;
+  if (c->n + add > c->cap) return fail(c, MPIC_ERR_CAPACITY, "Cannot create that number of particles, no fill performed");
+  FillArgs a{};
+  a.m = c->m;
+  a.P = c->P[c->cur];
+  a.offset = c->n;
+  a.nppe = nppe;
+  a.type = type;
+  a.density = density;
+  for (int d = 0; d < 3; ++d) a.vel[d] = vel ? vel[d] : 0.0;
+  a.temp = temp;
+  a.pert_amp = pert_amp;
+  a.pert_k = (2.0 * M_PI) / (c->m.hi[0] - c->m.lo[0]);
+  a.mass = c->cfg.species_m[type];
+  a.seed_offset = seed_offset;
+  HIPCHK(c, launch_fill(a, c->stream));
+  c->n += add;
+  c->ke_max_prev = 0;
+  int rc = invalidate_deposit(c);
+  if (rc) return rc;
+  return update_charge_scale(c);
+}
+
+int mpic_weight_charge(mpic_ctx* c) {
+  if (!c) return MPIC_ERR_ARG;
+  tstart(c, T_DEPOSIT);
+  int rc = do_weight_charge(c);
+  tstop(c, T_DEPOSIT);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
+  return MPIC_OK;
+}
+
+int mpic_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
+  if (!c || !(tol >= 0) || max_iter < 0) return fail(c, MPIC_ERR_ARG, "mpic_solve: bad arguments");
+  tstart(c, T_SOLVE);
+  int rc = do_solve(c, tol, max_iter, iters, relres);
+  tstop(c, T_SOLVE);
+  (void)hipStreamSynchronize(c->stream);
+  c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
+  return rc;
+}
+
+int mpic_weight_efield(mpic_ctx* c) {
+  if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_weight_efield: no particle list");
+  int rc = ensure_transients(c);
+  if (rc) return rc;
+  HIPCHK(c, launch_gather(push_args(c), c->stream));
+  return MPIC_OK;
+}
+
+int mpic_accel(mpic_ctx* c, double dt, double scale) {
+  if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_accel: no particle list");
+  int rc = ensure_transients(c);
+  if (rc) return rc;
+  PushArgs a = push_args(c);
+  a.dt = dt;
+  a.scale = scale;
+  HIPCHK(c, launch_accel(a, c->stream));
+  return MPIC_OK;
+}
+
+int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
+  if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_move: no particle list");
+  int rc = ensure_transients(c);
+  if (rc) return rc;
+  if ((rc = invalidate_deposit(c))) return rc;
+  PushArgs a = push_args(c);
+  a.dt = dt;
+  tstart(c, T_PUSH);
+  HIPCHK(c, launch_move(a, c->stream));
+  tstop(c, T_PUSH);
+  PushPartial r{};
+  if (c->n > 0 && (rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+  c->ms[T_PUSH] = c->n > 0 ? telapsed(c, T_PUSH) : 0;
+  if (st) *st = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
+  if (r.killed > 0 && (rc = do_sort(c, nullptr))) return rc;  // cleanup_list, particle_list.cpp:70-73
+  if (r.capped > 0) return fail(c, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
+  return MPIC_OK;
+}
+
+int mpic_compute_ke(mpic_ctx* c, double* ke) {
+  if (!c || !ke) return MPIC_ERR_ARG;
+  if (c->n == 0) {
+    *ke = 0;
+    return MPIC_OK;
+  }
+  const DevParticles& P = c->P[c->cur];
+  PushPartial r{};
+  HIPCHK(c, launch_ke(P, c->n, c->species, 0, 0, c->partial, c->stream));
+  int rc = finish_partials(c, nblocks(c->n), &r);
+  if (rc) return rc;
+  double mx;
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  const int S = scale_for_bound((double)c->n * mx);
+  HIPCHK(c, launch_ke(P, c->n, c->species, S, 1, c->partial, c->stream));
+  if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+  *ke = from_fixed(fx128{r.ke_lo, r.ke_hi}, S);
+  c->ke_max_prev = mx;
+  return MPIC_OK;
+}
+
+int mpic_sort_by_element(mpic_ctx* c, int32_t* start_opt) {
+  if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_sort_by_element: no particle list");
+  return do_sort(c, start_opt);
+}
+
+int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
+  if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_time_step: no particle list");
+  int rc;
+  mpic_step_stats st{};
+  // 1. charge of the current positions (PIC.cpp:250)
+  tstart(c, T_DEPOSIT);
+  if ((rc = do_weight_charge(c))) return rc;
+  tstop(c, T_DEPOSIT);
+  // 2. field solve, warm start from the previous potential (PIC.cpp:251)
+  tstart(c, T_SOLVE);
+  rc = do_solve(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres);
+  tstop(c, T_SOLVE);
+  if (rc) return rc;
+  // 3. fused gather + kick + walk + deposit of the new positions + KE (PIC.cpp:252-258)
+  if (!(c->ke_max_prev > 0) && c->n > 0) {
+    PushPartial r{};
+    HIPCHK(c, launch_ke(c->P[c->cur], c->n, c->species, 0, 0, c->partial, c->stream));
+    if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+    std::memcpy(&c->ke_max_prev, &r.ke_max_bits, 8);
+  }
+  PushArgs a = push_args(c);
+  a.dt = dt;
+  a.S_ke = scale_for_bound((double)c->n * c->ke_max_prev * 256.0);
+  tstart(c, T_PUSH);
+  tstart(c, T_PUSH_KERNEL);
+  HIPCHK(c, launch_push_fused(a, c->stream));
+  tstop(c, T_PUSH_KERNEL);
+  PushPartial r{};
+  if (c->n > 0) {
+    if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+  }
+  tstop(c, T_PUSH);
+  c->acc_deposited = true;
+  double mx;
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  const int64_t n_before = c->n;
+  tstart(c, T_KE);
+  const int S_safe = scale_for_bound((double)n_before * mx);
+  if (a.S_ke > S_safe) {
+    // The provisional scale could have overflowed: sum again at a safe scale.
+    PushPartial k{};
+    HIPCHK(c, launch_ke(c->P[c->cur], c->n, c->species, S_safe, 1, c->partial, c->stream));
+    if ((rc = finish_partials(c, nblocks(c->n), &k))) return rc;
+    st.kinetic_energy = from_fixed(fx128{k.ke_lo, k.ke_hi}, S_safe);
+  } else {
+    st.kinetic_energy = from_fixed(fx128{r.ke_lo, r.ke_hi}, a.S_ke);
+  }
+  tstop(c, T_KE);
+  c->ke_max_prev = mx;
+  st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
+  c->steps++;
+  // 4. remove killed particles; periodic counting sort by element
+  c->ms[T_SORT] = 0;
+  if (r.killed > 0 || (c->cfg.sort_period > 0 && c->steps % c->cfg.sort_period == 0))
+    if ((rc = do_sort(c, nullptr))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
+  c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
+  c->ms[T_PUSH] = telapsed(c, T_PUSH);
+  c->ms[T_PUSH_KERNEL] = telapsed(c, T_PUSH_KERNEL);
+  c->ms[T_KE] = telapsed(c, T_KE);
+  st.num_particles = c->n;
+  st.charge_scale_bits = c->S;
+  st.ms_deposit = c->ms[T_DEPOSIT];
+  st.ms_solve = c->ms[T_SOLVE];
+  st.ms_push = c->ms[T_PUSH];
+  st.ms_sort = c->ms[T_SORT];
+  st.ms_ke = c->ms[T_KE];
+  if (out) *out = st;
+  if (r.capped > 0) return fail(c, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
+  return MPIC_OK;
+}
+
+int mpic_get_rhs(mpic_ctx* c, double* rhs) {
+  if (!c || !rhs) return MPIC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(rhs, c->rhs, (size_t)c->m.nn * 8, hipMemcpyDeviceToHost));
+  return MPIC_OK;
+}
+
+int mpic_get_phi(mpic_ctx* c, double* phi) {
+  if (!c || !phi) return MPIC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(phi, c->phi, (size_t)c->m.nn * 8, hipMemcpyDeviceToHost));
+  return MPIC_OK;
+}
+
+int mpic_set_phi(mpic_ctx* c, const double* phi) {
+  if (!c || !phi) return MPIC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->phi, phi, (size_t)c->m.nn * 8, hipMemcpyHostToDevice));
+  return MPIC_OK;
+}
+
+int mpic_get_efield(mpic_ctx* c, double* E, int64_t stride) {
+  if (!c || !E || stride < c->n || !c->E) return fail(c, MPIC_ERR_STATE, "mpic_get_efield: no field gathered");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; ++d)
+    HIPCHK(c, hipMemcpy(E + d * stride, c->E + d * c->cap, c->n * 8, hipMemcpyDeviceToHost));
+  return MPIC_OK;
+}
+
+int mpic_charge_scale(mpic_ctx* c, int* bits) {
+  if (!c || !bits) return MPIC_ERR_ARG;
+  *bits = c->S;
+  return MPIC_OK;
+}
+
+int mpic_kernel_time(mpic_ctx* c, int which, double* ms) {
+  if (!c || !ms) return MPIC_ERR_ARG;
+  switch (which) {
+    case 0: *ms = c->ms[T_PUSH_KERNEL]; break;
+    case 1: *ms = c->ms[T_DEPOSIT]; break;
+    case 3: *ms = c->last_cg_iters > 0 ? c->ms[T_SOLVE] / c->last_cg_iters : 0; break;
+    case 4: *ms = c->ms[T_SORT]; break;
+    default: return MPIC_ERR_ARG;
+  }
+  return MPIC_OK;
+}
+
+}  // extern "C"

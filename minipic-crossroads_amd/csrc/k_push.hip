@@ -1,0 +1,454 @@
+// Particle kernels of the MiniPIC timestep on gfx950:
+//   fused push  = E gather + Boris kick + barycentric tet walk + charge deposit of
+//                 the new position + kinetic energy, one thread per particle;
+//   split phases = the same arithmetic as separate launches (weight_Efield,
+//                 accel, move) for the reference's phase-by-phase API.
+// All FP64 expressions keep the reference's association order and are compiled
+// without FMA contraction, so results are bit-identical to the CPU restatement.
+#include "mpic_internal.hpp"
+#include "mpic_device.hpp"
+
+namespace mpic {
+namespace {
+
+// Gather, PIC.cpp:162-184: weights from getWeights (mesh.hpp:170-176),
+// E_d = sum_i sum_j (phi_i*w_j)*G[e][i]_d.
+__device__ __forceinline__ void gather_E(const DevMesh& m, const double* __restrict__ phi, int e, const double R[3],
+                                         double E[3]) {
+  const double wt[4] = {1 - R[0] - R[1] - R[2], R[0], R[1], R[2]};
+  const int4 nd = m.conn[e];
+  const double dof[4] = {phi[nd.x], phi[nd.y], phi[nd.z], phi[nd.w]};
+  double G[12];
+  load12(m.grad, e, G);
+  E[0] = E[1] = E[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) E[d] += dof[i] * wt[j] * G[i * 3 + d];
+}
+
+// Boris push with B = 0, particle_move.hpp:364-392; v_ref = J^-1 v.
+__device__ __forceinline__ void kick(const double* Ji, double V[3], const double E[3], double qm, double dt,
+                                     double scale, double VR[3]) {
+  const double B[3] = {0.0, 0.0, 0.0};
+  const double sdt = dt * scale;
+  double t[3];
+  t[0] = qm * 0.5 * B[0];
+  t[1] = qm * 0.5 * B[1];
+  t[2] = qm * 0.5 * B[2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) V[d] += sdt * E[d] * qm * 0.5;
+  double tmp[3] = {V[0], V[1], V[2]};
+  tmp[0] += V[1] * t[2] - V[2] * t[1];
+  tmp[1] += V[2] * t[0] - V[0] * t[2];
+  tmp[2] += V[0] * t[1] - V[1] * t[0];
+  const double tsq = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) t[d] *= 2. / (1 + tsq);
+  V[0] += tmp[1] * t[2] - tmp[2] * t[1];
+  V[1] += tmp[2] * t[0] - tmp[0] * t[2];
+  V[2] += tmp[0] * t[1] - tmp[1] * t[0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) V[d] += sdt * E[d] * qm * 0.5;
+  p2r_vec(Ji, V, VR);
+}
+
+struct WalkCount {
+  int crossings, capped, nonpos, reflections;
+};
+
+// MoveParticle::operator() + find_intersection, particle_move.hpp:103-124,136-225.
+// Returns false when the particle is killed on a wall.
+__device__ __forceinline__ bool walk(const DevMesh& m, int& e, double X[3], double R[3], double V[3], double VR[3],
+                                     double dt, int kill, WalkCount& wc) {
+  double tr = 1.0;
+  int iters = 0;
+  while (tr > 0) {
+    if (iters++ >= WALK_CAP) {
+      wc.capped++;
+      break;
+    }
+    const double ldt = dt * tr;
+    double s = 1.0;
+    int face = -1;
+    double dxr[3], xf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      dxr[d] = ldt * VR[d];
+      xf[d] = R[d] + dxr[d];
+    }
+    if (VR[2] < 0 && xf[2] < 0) {
+      face = 3;
+      s = -R[2] / (dxr[2]);
+    }
+    if (VR[1] < 0 && xf[1] < 0) {
+      const double sfy = -R[1] / (dxr[1]);
+      if (sfy < s) {
+        face = 0;
+        s = sfy;
+      }
+    }
+    if (VR[0] < 0 && xf[0] < 0) {
+      const double sfx = -R[0] / (dxr[0]);
+      if (sfx < s) {
+        face = 2;
+        s = sfx;
+      }
+    }
+    const double xfs = xf[0] + xf[1] + xf[2];
+    const double vrs = VR[0] + VR[1] + VR[2];
+    if (vrs > 0 && xfs > 1) {
+      const double sfs = (1 - R[0] - R[1] - R[2]) / (dxr[0] + dxr[1] + dxr[2]);
+      if (sfs < s) {
+        face = 1;
+        s = sfs;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      X[d] = X[d] + s * ldt * V[d];
+      R[d] = R[d] + s * ldt * VR[d];
+    }
+    if (s < 1) {
+      const int4 nb = m.nbrs[e];
+      const int e2 = face == 0 ? nb.x : face == 1 ? nb.y : face == 2 ? nb.z : nb.w;
+      if (e2 >= 0) {
+        e = e2;
+        wc.crossings++;
+        double g[12];
+        load12(m.geo, e, g);  // inverse Jacobian + node 0 of the new element
+        // physToReference, mesh.hpp:197-211
+        const double dl[3] = {X[0] - g[9], X[1] - g[10], X[2] - g[11]};
+        double dr[3];
+        R[0] = R[1] = R[2] = 0.0;
+        p2r_vec(g, dl, dr);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) R[d] += dr[d];
+        p2r_vec(g, V, VR);
+      } else {
+        if (kill) return false;
+        wc.reflections++;
+        const double* nrm = m.normals + (size_t)e * 12 + face * 3;
+        const double n0 = nrm[0], n1 = nrm[1], n2 = nrm[2];
+        double vn = 0;
+        vn += V[0] * n0;
+        vn += V[1] * n1;
+        vn += V[2] * n2;
+        V[0] -= 2 * vn * n0;
+        V[1] -= 2 * vn * n1;
+        V[2] -= 2 * vn * n2;
+        double g[12];
+        load12(m.geo, e, g);
+        p2r_vec(g, V, VR);
+      }
+    }
+    if (!(s > 0)) wc.nonpos++;
+    tr -= s * tr;
+  }
+  return true;
+}
+
+// Per-node charge of one particle, PIC.cpp:121-131, in fixed point.
+__device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double w, double q, int S, fx128 c[4]) {
+  const double tq = w * q * m.sixodet[e];
+  double W[16];
+  load16(m.wbv, e, W);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double acc = 0;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) acc += -tq * W[i * 4 + cc];
+    c[i] = to_fixed(acc, S);
+  }
+}
+
+__device__ __forceinline__ double ke_term(double w, double mass, const double V[3]) {
+  const double tm = w * mass;
+  double vs = 0;
+  vs += V[0] * V[0];
+  vs += V[1] * V[1];
+  vs += V[2] * V[2];
+  return 0.5 * tm * vs;
+}
+
+// Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x].
+__device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, double kemax, const WalkCount& wc,
+                                              int killed) {
+  __shared__ PushPartial sh[BLOCK / 64];
+  unsigned long long klo = ke.lo;
+  long long khi = ke.hi;
+  unsigned long long kmax = __double_as_longlong(kemax);
+  long long c0 = wc.crossings, c1 = killed, c2 = wc.capped, c3 = wc.nonpos, c4 = wc.reflections;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    unsigned long long olo = __shfl_xor(klo, off);
+    long long ohi = __shfl_xor(khi, off);
+    unsigned long long nlo = klo + olo;
+    khi = khi + ohi + (nlo < klo ? 1 : 0);
+    klo = nlo;
+    unsigned long long om = __shfl_xor(kmax, off);
+    kmax = om > kmax ? om : kmax;
+    c0 += __shfl_xor(c0, off);
+    c1 += __shfl_xor(c1, off);
+    c2 += __shfl_xor(c2, off);
+    c3 += __shfl_xor(c3, off);
+    c4 += __shfl_xor(c4, off);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[wave] = PushPartial{klo, khi, kmax, c0, c1, c2, c3, c4};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    PushPartial r = sh[0];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      fx128 s = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
+      r.ke_lo = s.lo;
+      r.ke_hi = s.hi;
+      r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
+      r.crossings += sh[w].crossings;
+      r.killed += sh[w].killed;
+      r.capped += sh[w].capped;
+      r.nonpos += sh[w].nonpos;
+      r.reflections += sh[w].reflections;
+    }
+    out[blockIdx.x] = r;
+  }
+}
+
+struct KArgs {
+  DevMesh m;
+  DevParticles P;
+  int64_t n;
+  const double* phi;
+  double dt, scale;
+  int kill, S_charge, S_ke;
+  fx128* acc_tet;
+  PushPartial* partial;
+  double* E;
+  double* vr;
+  const double* sp;  // [48]: q[16], m[16], q/m[16]
+};
+
+__global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t cap = a.P.cap;
+  int enew = -1, killed = 0;
+  double w = 0, V[3] = {0, 0, 0};
+  int t = 0;
+  WalkCount wc{0, 0, 0, 0};
+  if (p < a.n) {
+    int e = a.P.elem[p];
+    if (e >= 0) {
+      double X[3], R[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        X[d] = a.P.x[d * cap + p];
+        R[d] = a.P.xr[d * cap + p];
+        V[d] = a.P.v[d * cap + p];
+      }
+      t = a.P.type[p];
+      w = a.P.w[p];
+      double E[3];
+      gather_E(a.m, a.phi, e, R, E);
+      double g[12];
+      load12(a.m.geo, e, g);
+      double VR[3];
+      kick(g, V, E, a.sp[32 + t], a.dt, a.scale, VR);
+      const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.P.x[d * cap + p] = X[d];
+        a.P.xr[d * cap + p] = R[d];
+        a.P.v[d * cap + p] = V[d];
+      }
+      enew = alive ? e : -1;
+      killed = alive ? 0 : 1;
+      a.P.elem[p] = enew;
+    }
+  }
+  fx128 c[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (enew >= 0) particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
+  deposit_segmented(enew, c, a.acc_tet);
+  double term = 0;
+  fx128 k{0, 0};
+  if (enew >= 0) {
+    term = ke_term(w, a.sp[16 + t], V);
+    k = to_fixed(term, a.S_ke);
+  }
+  block_partial(a.partial, k, term, wc, killed);
+}
+
+// weight_Efield: E of every live particle (PIC.cpp:186-189).
+__global__ __launch_bounds__(BLOCK) void k_gather(KArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const int e = a.P.elem[p];
+  if (e < 0) return;
+  const int64_t cap = a.P.cap;
+  double R[3], E[3];
+  for (int d = 0; d < 3; ++d) R[d] = a.P.xr[d * cap + p];
+  gather_E(a.m, a.phi, e, R, E);
+  for (int d = 0; d < 3; ++d) a.E[d * cap + p] = E[d];
+}
+
+// AccelParticlesTag: kick v from the stored E and store v_ref.
+__global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (p >= a.n) return;
+  const int e = a.P.elem[p];
+  if (e < 0) return;
+  const int64_t cap = a.P.cap;
+  double V[3], E[3], VR[3];
+  for (int d = 0; d < 3; ++d) {
+    V[d] = a.P.v[d * cap + p];
+    E[d] = a.E[d * cap + p];
+  }
+  double g[12];
+  load12(a.m.geo, e, g);
+  kick(g, V, E, a.sp[32 + a.P.type[p]], a.dt, a.scale, VR);
+  for (int d = 0; d < 3; ++d) {
+    a.P.v[d * cap + p] = V[d];
+    a.vr[d * cap + p] = VR[d];
+  }
+}
+
+// MoveParticle: the tet walk from the stored v and v_ref (no deposit).
+__global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t cap = a.P.cap;
+  WalkCount wc{0, 0, 0, 0};
+  int killed = 0;
+  if (p < a.n) {
+    int e = a.P.elem[p];
+    if (e >= 0) {
+      double X[3], R[3], V[3], VR[3];
+      for (int d = 0; d < 3; ++d) {
+        X[d] = a.P.x[d * cap + p];
+        R[d] = a.P.xr[d * cap + p];
+        V[d] = a.P.v[d * cap + p];
+        VR[d] = a.vr[d * cap + p];
+      }
+      const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+      for (int d = 0; d < 3; ++d) {
+        a.P.x[d * cap + p] = X[d];
+        a.P.xr[d * cap + p] = R[d];
+        a.P.v[d * cap + p] = V[d];
+      }
+      a.P.elem[p] = alive ? e : -1;
+      killed = alive ? 0 : 1;
+    }
+  }
+  block_partial(a.partial, fx128{0, 0}, 0.0, wc, killed);
+}
+
+// Deposit of the current positions (weight_charge, PIC.cpp:139-160).
+__global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, int64_t n, const double* sp, int S,
+                                                   fx128* acc_tet) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  int e = -1;
+  fx128 c[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (p < n) {
+    e = P.elem[p];
+    if (e >= 0) particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
+  }
+  deposit_segmented(e, c, acc_tet);
+}
+
+// KE: max term pass and sum pass (PIC.cpp:191-200).
+__global__ __launch_bounds__(BLOCK) void k_ke(DevParticles P, int64_t n, const double* sp, int S, int want_sum,
+                                              PushPartial* partial) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  double term = 0;
+  fx128 k{0, 0};
+  if (p < n && P.elem[p] >= 0) {
+    double V[3];
+    for (int d = 0; d < 3; ++d) V[d] = P.v[d * P.cap + p];
+    term = ke_term(P.w[p], sp[16 + P.type[p]], V);
+    if (want_sum) k = to_fixed(term, S);
+  }
+  block_partial(partial, k, term, WalkCount{0, 0, 0, 0}, 0);
+}
+
+// Final reduction of per-workgroup partials (one workgroup, integer sums).
+__global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, int nparts, PushPartial* out) {
+  fx128 ke{0, 0};
+  unsigned long long mx = 0;
+  long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  for (int i = threadIdx.x; i < nparts; i += BLOCK) {
+    const PushPartial q = part[i];
+    ke = fx_add(ke, fx128{q.ke_lo, q.ke_hi});
+    mx = q.ke_max_bits > mx ? q.ke_max_bits : mx;
+    c0 += q.crossings;
+    c1 += q.killed;
+    c2 += q.capped;
+    c3 += q.nonpos;
+    c4 += q.reflections;
+  }
+  __shared__ PushPartial sh[BLOCK];
+  sh[threadIdx.x] = PushPartial{ke.lo, ke.hi, mx, c0, c1, c2, c3, c4};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    PushPartial r = sh[0];
+    for (int w = 1; w < BLOCK; ++w) {
+      fx128 s = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
+      r.ke_lo = s.lo;
+      r.ke_hi = s.hi;
+      r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
+      r.crossings += sh[w].crossings;
+      r.killed += sh[w].killed;
+      r.capped += sh[w].capped;
+      r.nonpos += sh[w].nonpos;
+      r.reflections += sh[w].reflections;
+    }
+    *out = r;
+  }
+}
+
+KArgs kargs(const PushArgs& a) {
+  return KArgs{a.m, a.P, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge, a.S_ke, a.acc_tet, a.partial, a.E, a.vr,
+               a.species};
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+hipError_t launch_push_fused(const PushArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_push_fused, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  return hipGetLastError();
+}
+hipError_t launch_gather(const PushArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  return hipGetLastError();
+}
+hipError_t launch_accel(const PushArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_accel, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  return hipGetLastError();
+}
+hipError_t launch_move(const PushArgs& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_move, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  return hipGetLastError();
+}
+hipError_t launch_push_finish(const PushPartial* part, int nparts, PushPartial* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_push_finish, dim3(1), dim3(BLOCK), 0, s, part, nparts, out);
+  return hipGetLastError();
+}
+hipError_t launch_deposit(const DevMesh& m, const DevParticles& P, int64_t n, const double* species, int S,
+                          fx128* acc_tet, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_deposit, dim3(grid_for(n)), dim3(BLOCK), 0, s, m, P, n, species, S, acc_tet);
+  return hipGetLastError();
+}
+hipError_t launch_ke(const DevParticles& P, int64_t n, const double* species, int S, int want_sum,
+                     PushPartial* partial, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ke, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, n, species, S, want_sum, partial);
+  return hipGetLastError();
+}
+
+}  // namespace mpic

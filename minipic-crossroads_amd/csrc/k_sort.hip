@@ -1,0 +1,160 @@
+// Counting sort of the particle SoA by owning element: the build's replacement
+// for ParticleList::sort (Kokkos::BinSort, particle_list.cpp:90-116) and
+// compute_start_particles_by_element (particle_list.cpp:118-146).
+//   1. count[e] = particles in element e (wave-aggregated atomics);
+//   2. start = exclusive scan of count (start[ne] = live particles);
+//   3. scatter every live particle to start[e] + rank; dead particles
+//      (elem < 0) are dropped, which also compacts the list
+//      (DeleteParticles, particle_list.cpp:148-191).
+// As in the reference, the order inside one element is not specified; the
+// identity key (id) travels with the particle.
+#include "mpic_internal.hpp"
+#include "mpic_device.hpp"
+
+namespace mpic {
+namespace {
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
+// Per-lane run info for contiguous equal keys in a wavefront.
+struct Run {
+  bool head;
+  int head_lane;
+  int len;  // valid in the head lane
+};
+__device__ __forceinline__ Run wave_runs(int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = (lane == 0) || (prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int head_lane = 63 - __clzll(heads & le);
+  const unsigned long long after = heads & ~le;
+  const int next = after ? (__ffsll((long long)after) - 1) : 64;
+  return Run{head, head_lane, next - lane};
+}
+
+__global__ __launch_bounds__(BLOCK) void k_count(const int32_t* __restrict__ elem, int64_t n, int32_t* count) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int e = p < n ? elem[p] : -1;
+  const Run r = wave_runs(e);
+  if (r.head && e >= 0) atomicAdd(count + e, r.len);
+}
+
+// Exclusive scan, pass 1: tile totals.
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(const int32_t* __restrict__ in, int64_t n, int32_t* tile_sum) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) s += in[base + k];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  __shared__ int ws[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Block-wide exclusive scan of one int per thread; returns the block total in *total.
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  __shared__ int ws[BLOCK / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(inc, off);
+    if (lane >= off) inc += o;
+  }
+  __syncthreads();
+  if (lane == 63) ws[wave] = inc;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wave; ++w) wbase += ws[w];
+  *total = ws[0] + ws[1] + ws[2] + ws[3];
+  return wbase + inc - v;
+}
+
+// Pass 2: scan of the tile totals in one workgroup (in place, exclusive).
+__global__ __launch_bounds__(BLOCK) void k_scan_top(int32_t* tile_sum, int ntiles) {
+  const int per = (ntiles + BLOCK - 1) / BLOCK;
+  const int b = threadIdx.x * per;
+  int s = 0;
+  for (int k = 0; k < per; ++k)
+    if (b + k < ntiles) s += tile_sum[b + k];
+  int total;
+  int run = block_excl_scan(s, &total);
+  for (int k = 0; k < per; ++k)
+    if (b + k < ntiles) {
+      const int t = tile_sum[b + k];
+      tile_sum[b + k] = run;
+      run += t;
+    }
+}
+
+// Pass 3: exclusive scan inside each tile plus the tile offset.
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict__ in, int64_t n,
+                                                      const int32_t* __restrict__ tile_off, int32_t* out,
+                                                      int32_t* cursor, int64_t ncursor) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int vals[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    vals[k] = (base + k < n) ? in[base + k] : 0;
+    s += vals[k];
+  }
+  int total;
+  int run = block_excl_scan(s, &total) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) {
+      out[base + k] = run;
+      if (base + k < ncursor) cursor[base + k] = run;
+    }
+    run += vals[k];
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(DevParticles src, DevParticles dst, int64_t n,
+                                                   int32_t* __restrict__ cursor) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int e = p < n ? src.elem[p] : -1;
+  const Run r = wave_runs(e);
+  int base = 0;
+  if (r.head && e >= 0) base = atomicAdd(cursor + e, r.len);
+  base = __shfl(base, r.head_lane);
+  if (e < 0) return;
+  const int64_t q = base + ((threadIdx.x & 63) - r.head_lane);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dst.x[d * dst.cap + q] = src.x[d * src.cap + p];
+    dst.xr[d * dst.cap + q] = src.xr[d * src.cap + p];
+    dst.v[d * dst.cap + q] = src.v[d * src.cap + p];
+  }
+  dst.w[q] = src.w[p];
+  dst.elem[q] = e;
+  dst.type[q] = src.type[p];
+  dst.id[q] = src.id[p];
+}
+
+}  // namespace
+
+hipError_t launch_sort(const DevMesh& m, const DevParticles& src, const DevParticles& dst, int64_t n,
+                       SortBuffers& sb, hipStream_t s) {
+  const int64_t ne1 = (int64_t)m.ne + 1;
+  hipError_t err = launch_zero(sb.count, (size_t)((ne1 + 1) / 2) * 8, s);
+  if (err != hipSuccess) return err;
+  const unsigned gp = (unsigned)((n + BLOCK - 1) / BLOCK);
+  if (n > 0) hipLaunchKernelGGL(k_count, dim3(gp), dim3(BLOCK), 0, s, src.elem, n, sb.count);
+  const int ntiles = (int)((ne1 + SCAN_TILE - 1) / SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, sb.blocksum, ntiles);
+  hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum, sb.start, sb.cursor,
+                     (int64_t)m.ne);
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(gp), dim3(BLOCK), 0, s, src, dst, n, sb.cursor);
+  return hipGetLastError();
+}
+
+}  // namespace mpic

@@ -1,0 +1,192 @@
+// Internal declarations of the MI355X MiniPIC library: device data layout,
+// exact fixed-point arithmetic and launch wrappers shared by the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "minipic_c.h"
+
+namespace mpic {
+
+constexpr int BLOCK = 256;         // threads per workgroup for particle and row kernels
+constexpr int WALK_CAP = 1000;     // crossings per particle per step (the reference has no cap)
+constexpr int ELL_SLICE = 64;      // rows per ELL slice = one wavefront
+
+// 128-bit two's-complement fixed-point value (lo word first).
+struct fx128 {
+  unsigned long long lo;
+  long long hi;
+};
+
+// ---------------------------------------------------------------------------
+// Device view of the mesh and discretisation (all FP64 unless noted).
+// ---------------------------------------------------------------------------
+struct DevMesh {
+  int32_t nn, ne;
+  const double* geo;       // [ne][12]  inverse Jacobian (row-major 3x3) then node-0 coordinates
+  const double* grad;      // [ne][12]  physical P1 gradients G[e][i][d]
+  const int4* conn;        // [ne]      element -> 4 node ids
+  const int4* nbrs;        // [ne]      face f -> neighbour element, -1 wall
+  const double* normals;   // [ne][12]  unit face normals
+  const double* sixodet;   // [ne]      6/detJ
+  const double* detj;      // [ne]      det J
+  const double* wbv;       // [ne][16]  weighted basis values wbv[i][c]
+  const double* nodes;     // [nn][3]
+  const uint8_t* bnd;      // [nn]      Dirichlet flag
+  const int32_t* adj_ptr;  // [nn+1]    node -> slots e*4+k of elements touching it
+  const int32_t* adj;      // [4*ne]
+  const fx128* bgfix;      // [nn]      background load vector in fixed point at the current scale
+  // Stiffness matrix in sliced ELL (64-row slices, entries column-major inside
+  // a slice so a wavefront reads one entry of 64 rows per load).  Each row keeps
+  // the CSR entry order; padding entries are (col = row, val = +0).
+  int32_t nslices;
+  const int64_t* ell_off;  // [nslices+1] entry offset of each slice
+  const int32_t* ell_len;  // [nslices]   entries per row in the slice
+  const int32_t* ell_col;
+  const double* ell_val;
+  double lo[3], hi[3];
+};
+
+// FP64 particle SoA on the device.
+struct DevParticles {
+  double* x;      // [3][cap]
+  double* xr;     // [3][cap]
+  double* v;      // [3][cap]
+  double* w;      // [cap]
+  int32_t* elem;  // [cap], -1 dead
+  uint8_t* type;  // [cap]
+  uint32_t* id;   // [cap]
+  int64_t cap;
+};
+
+struct Species {
+  double q[MPIC_MAX_SPECIES];
+  double m[MPIC_MAX_SPECIES];
+  double qm[MPIC_MAX_SPECIES];
+};
+
+// Per-workgroup partial results of the fused push (integer sums: order-free).
+struct PushPartial {
+  unsigned long long ke_lo;
+  long long ke_hi;
+  unsigned long long ke_max_bits;  // max KE term (non-negative double bits)
+  long long crossings, killed, capped, nonpos, reflections;
+};
+
+struct CGState {
+  int done, iters, converged, pad;
+  double rho0, rho, rho_old, alpha, beta, tol2;
+  unsigned int cnt_init, cnt_k1, cnt_k2, pad2;
+  int max_iter, pad3;
+};
+
+// ---------------------------------------------------------------------------
+// Exact fixed point: value v -> round-to-nearest-even(v * 2^S) as a 128-bit
+// integer.  Sums of such integers are exact and independent of order.
+// Back-conversion: |a| = hi*2^64 + lo, (double)hi * 2^64 + (double)lo, * 2^-S.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline fx128 fx_neg(fx128 a) {
+  unsigned long long lo = ~a.lo + 1ull;
+  unsigned long long hi = ~(unsigned long long)a.hi + (lo == 0ull ? 1ull : 0ull);
+  return {lo, (long long)hi};
+}
+__host__ __device__ inline fx128 fx_add(fx128 a, fx128 b) {
+  unsigned long long lo = a.lo + b.lo;
+  unsigned long long c = lo < a.lo ? 1ull : 0ull;
+  return {lo, (long long)((unsigned long long)a.hi + (unsigned long long)b.hi + c)};
+}
+__host__ __device__ inline fx128 to_fixed(double v, int S) {
+  double r = rint(ldexp(v, S));
+  bool neg = r < 0;
+  double mg = fabs(r);
+  double hd = floor(mg * 5.421010862427522170e-20);  // * 2^-64
+  double ld = mg - hd * 18446744073709551616.0;
+  fx128 a{(unsigned long long)ld, (long long)(unsigned long long)hd};
+  return neg ? fx_neg(a) : a;
+}
+__host__ __device__ inline double from_fixed(fx128 a, int S) {
+  bool neg = a.hi < 0;
+  if (neg) a = fx_neg(a);
+  double d = (double)(unsigned long long)a.hi * 18446744073709551616.0 + (double)a.lo;
+  d = ldexp(d, -S);
+  return neg ? -d : d;
+}
+// Scale so that any sum bounded by `bound` stays below 2^124 in fixed point.
+inline int scale_for_bound(double bound) {
+  if (!(bound > 0) || !std::isfinite(bound)) return 100;
+  int e = std::ilogb(bound) + 1;
+  int S = 124 - e;
+  if (S > 1000) S = 1000;
+  if (S < -1000) S = -1000;
+  return S;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (one .hip file each).  All are stream-ordered and return
+// hipError_t of the launch.
+// ---------------------------------------------------------------------------
+struct PushArgs {
+  DevMesh m;
+  DevParticles P;
+  int64_t n;
+  const double* phi;
+  double dt, scale;
+  int kill;
+  int S_charge, S_ke;
+  fx128* acc_tet;          // [ne*4] charge accumulators (fused deposit), may be null
+  PushPartial* partial;    // [gridDim.x]
+  double* E;               // transient [3][cap] for the split entry points
+  double* vr;              // transient [3][cap]
+  const double* species;   // device [48]: q[16], m[16], q/m[16]
+};
+
+hipError_t launch_push_fused(const PushArgs& a, hipStream_t s);   // gather+kick+walk+deposit+KE
+hipError_t launch_gather(const PushArgs& a, hipStream_t s);       // weight_Efield -> E
+hipError_t launch_accel(const PushArgs& a, hipStream_t s);        // kick from E -> v, vr
+hipError_t launch_move(const PushArgs& a, hipStream_t s);         // walk from v, vr
+hipError_t launch_push_finish(const PushPartial* part, int nparts, PushPartial* out, hipStream_t s);
+// KE terms over live particles into per-workgroup partials (want_sum = 0: max only).
+hipError_t launch_ke(const DevParticles& P, int64_t n, const double* species, int S, int want_sum,
+                     PushPartial* partial, hipStream_t s);
+
+hipError_t launch_deposit(const DevMesh& m, const DevParticles& P, int64_t n, const double* species, int S,
+                          fx128* acc_tet, hipStream_t s);
+hipError_t launch_node_charge(const DevMesh& m, fx128* acc_tet, int S, double* rhs, hipStream_t s);
+hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* species, unsigned long long* out2,
+                               hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
+struct CGBuffers {
+  double *x, *b, *r, *p0, *p1, *q;
+  double *part_bb, *part_rr, *part_pq;
+  CGState* st;
+  int nchunks;
+};
+hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_iter, hipStream_t s);
+hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, hipStream_t s);
+
+struct SortBuffers {
+  int32_t* count;    // [ne+1]
+  int32_t* start;    // [ne+1]
+  int32_t* cursor;   // [ne]
+  int32_t* blocksum; // scan scratch
+  int scan_blocks;
+};
+hipError_t launch_sort(const DevMesh& m, const DevParticles& src, const DevParticles& dst, int64_t n,
+                       SortBuffers& sb, hipStream_t s);
+
+struct FillArgs {
+  DevMesh m;
+  DevParticles P;
+  int64_t offset;
+  int nppe, type;
+  double density, vel[3], temp, pert_amp, pert_k, mass;
+  uint64_t seed_offset;
+};
+hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+
+}  // namespace mpic

@@ -1,0 +1,305 @@
+"""ctypes binding of libminipic.so (the C ABI of include/minipic_c.h).
+
+This is the Python host-side mirror used by the tests and bench.py; C++ users
+use include/minipic/*.hpp.  The library must be built (make -C this folder);
+loading fails loudly if it is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libminipic.so")
+
+OK = 0
+ERR_ARG, ERR_CAPACITY, ERR_NOT_CONVERGED, ERR_HIP, ERR_WALK_CAP, ERR_STATE, ERR_NO_DEVICE = -1, -2, -3, -4, -5, -6, -7
+FORM_FP64, FORM_MIXED = 0, 1
+MAX_SPECIES = 16
+
+# Every extern "C" symbol declared in include/minipic_c.h.
+EXPORTS = [
+    "mpic_config_default", "mpic_last_error", "mpic_mesh_brick", "mpic_mesh_set_background", "mpic_mesh_describe",
+    "mpic_mesh_free", "mpic_create", "mpic_destroy", "mpic_set_form", "mpic_particles_reserve",
+    "mpic_particles_upload", "mpic_particles_download", "mpic_num_particles", "mpic_fill", "mpic_weight_charge",
+    "mpic_solve", "mpic_weight_efield", "mpic_accel", "mpic_move", "mpic_compute_ke", "mpic_sort_by_element",
+    "mpic_time_step", "mpic_get_rhs", "mpic_get_phi", "mpic_set_phi", "mpic_get_efield", "mpic_charge_scale",
+    "mpic_kernel_time",
+]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("num_elems", C.c_int32), ("nnz", C.c_int64),
+                ("nodes", C.c_void_p), ("elem_nodes", C.c_void_p), ("elem_nbrs", C.c_void_p),
+                ("normals", C.c_void_p), ("inv_jacobian", C.c_void_p), ("det_jacobian", C.c_void_p),
+                ("grad_basis", C.c_void_p), ("wbv", C.c_void_p), ("boundary", C.c_void_p),
+                ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("values", C.c_void_p),
+                ("background", C.c_void_p)]
+
+
+class Config(C.Structure):
+    _fields_ = [("form", C.c_int), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("kill_on_wall", C.c_int),
+                ("sort_period", C.c_int), ("num_species", C.c_int),
+                ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES)]
+
+
+class SoaHost(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("x_ref", C.c_void_p), ("v", C.c_void_p), ("elem", C.c_void_p),
+                ("weight", C.c_void_p), ("type", C.c_void_p), ("id", C.c_void_p), ("stride", C.c_int64)]
+
+
+class MoveStats(C.Structure):
+    _fields_ = [("crossings", C.c_int64), ("killed", C.c_int64), ("capped", C.c_int64),
+                ("nonpositive", C.c_int64), ("reflections", C.c_int64)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("cg_iters", C.c_int), ("cg_relres", C.c_double), ("kinetic_energy", C.c_double),
+                ("move", MoveStats), ("num_particles", C.c_int64), ("charge_scale_bits", C.c_int),
+                ("ms_deposit", C.c_double), ("ms_solve", C.c_double), ("ms_push", C.c_double),
+                ("ms_sort", C.c_double), ("ms_ke", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "move"}
+        d.update({k: getattr(self.move, k) for k, _ in MoveStats._fields_})
+        return d
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libminipic.so is not built: run make -C %s" % HERE)
+        L = C.CDLL(LIB_PATH)
+        P, i32, i64, d, u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
+        sig = {
+            "mpic_config_default": (None, [P]),
+            "mpic_last_error": (C.c_char_p, [P]),
+            "mpic_mesh_brick": (i32, [i32, i32, i32, i32, P, P, d, u64, P]),
+            "mpic_mesh_set_background": (i32, [P, d]),
+            "mpic_mesh_describe": (i32, [P, P]),
+            "mpic_mesh_free": (None, [P]),
+            "mpic_create": (i32, [P, P, P]),
+            "mpic_destroy": (None, [P]),
+            "mpic_set_form": (i32, [P, i32]),
+            "mpic_particles_reserve": (i32, [P, i64]),
+            "mpic_particles_upload": (i32, [P, P, i64]),
+            "mpic_particles_download": (i32, [P, P, P]),
+            "mpic_num_particles": (i32, [P, P]),
+            "mpic_fill": (i32, [P, i32, i32, d, P, d, u64, d]),
+            "mpic_weight_charge": (i32, [P]),
+            "mpic_solve": (i32, [P, d, i32, P, P]),
+            "mpic_weight_efield": (i32, [P]),
+            "mpic_accel": (i32, [P, d, d]),
+            "mpic_move": (i32, [P, d, P]),
+            "mpic_compute_ke": (i32, [P, P]),
+            "mpic_sort_by_element": (i32, [P, P]),
+            "mpic_time_step": (i32, [P, d, P]),
+            "mpic_get_rhs": (i32, [P, P]),
+            "mpic_get_phi": (i32, [P, P]),
+            "mpic_set_phi": (i32, [P, P]),
+            "mpic_get_efield": (i32, [P, P, i64]),
+            "mpic_charge_scale": (i32, [P, P]),
+            "mpic_kernel_time": (i32, [P, i32, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class MpicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("minipic error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_config(**kw):
+    cfg = Config()
+    lib().mpic_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Mesh:
+    """Host mesh + discretisation built by the library (Mesh/ES/PIC setup)."""
+
+    def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0,
+                 background=None):
+        self._lo = np.array(lo, dtype=np.float64)
+        self._hi = np.array(hi, dtype=np.float64)
+        h = C.c_void_p()
+        rc = lib().mpic_mesh_brick(nx, ny, nz, split, _p(self._lo), _p(self._hi), jitter, jitter_seed, C.byref(h))
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_last_error(None).decode())
+        self.h = h
+        if background is not None:
+            lib().mpic_mesh_set_background(self.h, background)
+        self.desc = MeshDesc()
+        lib().mpic_mesh_describe(self.h, C.byref(self.desc))
+        self.nn, self.ne, self.nnz = self.desc.num_nodes, self.desc.num_elems, self.desc.nnz
+
+    def array(self, name):
+        """numpy copy of one described array."""
+        nn, ne, nnz = self.nn, self.ne, self.nnz
+        spec = {"nodes": (C.c_double, (nn, 3)), "elem_nodes": (C.c_int32, (ne, 4)), "elem_nbrs": (C.c_int32, (ne, 4)),
+                "normals": (C.c_double, (ne, 4, 3)), "inv_jacobian": (C.c_double, (ne, 3, 3)),
+                "det_jacobian": (C.c_double, (ne,)), "grad_basis": (C.c_double, (ne, 4, 3)),
+                "wbv": (C.c_double, (ne, 4, 4)), "boundary": (C.c_uint8, (nn,)), "row_ptr": (C.c_int32, (nn + 1,)),
+                "col_idx": (C.c_int32, (nnz,)), "values": (C.c_double, (nnz,)), "background": (C.c_double, (nn,))}
+        ct, shape = spec[name]
+        ptr = getattr(self.desc, name)
+        if not ptr:
+            return None
+        cnt = int(np.prod(shape))
+        buf = (ct * cnt).from_address(ptr)
+        return np.ctypeslib.as_array(buf).reshape(shape).copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mpic_mesh_free(self.h)
+            self.h = None
+
+
+def desc_from_arrays(arrs):
+    """MeshDesc over caller-owned numpy arrays (keys as in the oracle's Mesh)."""
+    d = MeshDesc()
+    d.num_nodes = arrs["nodes"].shape[0]
+    d.num_elems = arrs["elem_nodes"].shape[0]
+    d.nnz = arrs["col"].shape[0]
+    d.nodes, d.elem_nodes, d.elem_nbrs = _p(arrs["nodes"]), _p(arrs["elem_nodes"]), _p(arrs["neighbors"])
+    d.normals, d.inv_jacobian, d.det_jacobian = _p(arrs["normals"]), _p(arrs["jinv"]), _p(arrs["detj"])
+    d.grad_basis, d.wbv, d.boundary = _p(arrs["grad"]), _p(arrs["wbv"]), _p(arrs["boundary"])
+    d.row_ptr, d.col_idx, d.values = _p(arrs["rowptr"]), _p(arrs["col"]), _p(arrs["val"])
+    d.background = _p(arrs.get("background"))
+    return d
+
+
+class Context:
+    """One problem on the GPU (the device half of DataWarehouse + PIC)."""
+
+    def __init__(self, desc, config=None):
+        self.cfg = config if config is not None else default_config()
+        h = C.c_void_p()
+        rc = lib().mpic_create(C.byref(desc), C.byref(self.cfg), C.byref(h))
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_last_error(None).decode())
+        self.h = h
+        self.nn, self.ne = desc.num_nodes, desc.num_elems
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_last_error(self.h).decode())
+        return rc
+
+    def reserve(self, cap):
+        self._chk(lib().mpic_particles_reserve(self.h, cap))
+
+    def upload(self, x, xr, v, elem, w, typ, ids=None, n=None):
+        stride = x.shape[1]
+        n = stride if n is None else n
+        s = SoaHost(_p(x), _p(xr), _p(v), _p(elem), _p(w), _p(typ), _p(ids), stride)
+        self._chk(lib().mpic_particles_upload(self.h, C.byref(s), n))
+
+    def download(self):
+        n = self.num_particles()
+        x, xr, v = np.zeros((3, n)), np.zeros((3, n)), np.zeros((3, n))
+        elem, w = np.zeros(n, np.int32), np.zeros(n)
+        typ, ids = np.zeros(n, np.int32), np.zeros(n, np.uint32)
+        s = SoaHost(_p(x), _p(xr), _p(v), _p(elem), _p(w), _p(typ), _p(ids), n)
+        cnt = C.c_int64()
+        self._chk(lib().mpic_particles_download(self.h, C.byref(s), C.byref(cnt)))
+        return dict(x=x, xr=xr, v=v, elem=elem, w=w, type=typ, id=ids)
+
+    def num_particles(self):
+        n = C.c_int64()
+        self._chk(lib().mpic_num_particles(self.h, C.byref(n)))
+        return n.value
+
+    def fill(self, nppe, ptype, density, vel=(0.0, 0.0, 0.0), temp=1.0, seed_offset=0, pert_amp=0.0):
+        v = np.array(vel, dtype=np.float64)
+        self._chk(lib().mpic_fill(self.h, nppe, ptype, density, _p(v), temp, seed_offset, pert_amp))
+
+    def weight_charge(self):
+        self._chk(lib().mpic_weight_charge(self.h))
+
+    def solve(self, tol=1e-10, max_iter=1000, check=True):
+        it, rr = C.c_int(), C.c_double()
+        rc = lib().mpic_solve(self.h, tol, max_iter, C.byref(it), C.byref(rr))
+        if check:
+            self._chk(rc)
+        return rc == OK, it.value, rr.value
+
+    def weight_efield(self):
+        self._chk(lib().mpic_weight_efield(self.h))
+
+    def accel(self, dt, scale=1.0):
+        self._chk(lib().mpic_accel(self.h, dt, scale))
+
+    def move(self, dt):
+        st = MoveStats()
+        self._chk(lib().mpic_move(self.h, dt, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in MoveStats._fields_}
+
+    def compute_ke(self):
+        ke = C.c_double()
+        self._chk(lib().mpic_compute_ke(self.h, C.byref(ke)))
+        return ke.value
+
+    def sort(self, want_start=False):
+        start = np.zeros(self.ne + 1, np.int32) if want_start else None
+        self._chk(lib().mpic_sort_by_element(self.h, _p(start)))
+        return start
+
+    def time_step(self, dt):
+        st = StepStats()
+        self._chk(lib().mpic_time_step(self.h, dt, C.byref(st)))
+        return st.as_dict()
+
+    def rhs(self):
+        a = np.zeros(self.nn)
+        self._chk(lib().mpic_get_rhs(self.h, _p(a)))
+        return a
+
+    def phi(self):
+        a = np.zeros(self.nn)
+        self._chk(lib().mpic_get_phi(self.h, _p(a)))
+        return a
+
+    def set_phi(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self._chk(lib().mpic_set_phi(self.h, _p(phi)))
+
+    def efield(self):
+        n = self.num_particles()
+        E = np.zeros((3, n))
+        self._chk(lib().mpic_get_efield(self.h, _p(E), n))
+        return E
+
+    def charge_scale(self):
+        s = C.c_int()
+        self._chk(lib().mpic_charge_scale(self.h, C.byref(s)))
+        return s.value
+
+    def kernel_time(self, which):
+        ms = C.c_double()
+        self._chk(lib().mpic_kernel_time(self.h, which, C.byref(ms)))
+        return ms.value
