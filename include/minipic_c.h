@@ -113,9 +113,10 @@ const char* mpic_last_error(const mpic_ctx* ctx); /* ctx may be NULL: last creat
 
 /* ---- host mesh (Mesh::Mesh mesh.cpp:78-714, ES::ES ES.cpp:47-213, PIC::PIC PIC.cpp:203-242) ---- */
 /* Brick of nx*ny*nz hexes on [lo,hi], split = 5 (reference, mesh.cpp:83) or 6
- * (Kuhn); interior nodes jittered by +-jitter*h (0 = none). */
+ * (Kuhn); interior nodes jittered by +-jitter*h (0 = none); node mapping 0-3 as
+ * the reference's mesh file field (mesh.cpp:85-89,763-792). */
 int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
-                    uint64_t jitter_seed, mpic_mesh** out);
+                    uint64_t jitter_seed, int mapping, mpic_mesh** out);
 /* Uniform background charge density f as an rhs load vector (ES::set_analytic_RHS with a constant). */
 int mpic_mesh_set_background(mpic_mesh* mesh, double f);
 /* Borrowed views of the mesh arrays (valid until mpic_mesh_free). */

@@ -46,6 +46,29 @@ constexpr double kGradRef[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}
 constexpr double kCubA = 0.1381966011250105, kCubB = 0.5854101966249685, kCubW = 1.0 / 24.0;
 constexpr double kCub[4][3] = {{kCubA, kCubA, kCubA}, {kCubB, kCubA, kCubA}, {kCubA, kCubB, kCubA}, {kCubA, kCubA, kCubB}};
 
+// Mesh::apply_mapping, mesh.cpp:763-792: smooth maps of the brick applied to
+// the node coordinates once topology and boundary sets exist (mesh.cpp:587-588).
+void map_node(double* x, int mapping) {
+  const double X = x[0], Y = x[1], Z = x[2];
+  switch (mapping) {
+    case 1: {
+      const double r = (1.0 - std::fabs(X)) * (1.0 - std::fabs(Y));
+      x[0] = X * std::cos(r) - Y * std::sin(r);
+      x[1] = X * std::sin(r) + Y * std::cos(r);
+    } break;
+    case 2:
+      x[0] = X * std::sqrt(1 - .5 * Y * Y - .5 * Z * Z + Y * Y * Z * Z / 3.);
+      x[1] = Y * std::sqrt(1 - .5 * X * X - .5 * Z * Z + X * X * Z * Z / 3.);
+      x[2] = Z * std::sqrt(1 - .5 * Y * Y - .5 * X * X + Y * Y * X * X / 3.);
+      break;
+    case 3:
+      x[0] = X < 0 ? -(X * X) : X * X;
+      break;
+    default:
+      break;
+  }
+}
+
 uint64_t jitter_next(uint64_t& s) {
   s = 6364136223846793005ull * s + 1442695040888963407ull;
   return s;
@@ -55,8 +78,10 @@ class BrickBuilder {
  public:
   explicit BrickBuilder(mpic_mesh& m) : m_(m) {}
 
-  bool run(double jitter, uint64_t jseed) {
+  bool run(double jitter, uint64_t jseed, int mapping) {
     make_nodes(jitter, jseed);
+    if (mapping != 0)
+      for (int32_t g = 0; g < m_.nn; ++g) map_node(&m_.nodes[(size_t)g * 3], mapping);
     make_elements();
     match_faces();
     if (!element_geometry()) return false;
@@ -248,8 +273,8 @@ class BrickBuilder {
 }  // namespace
 
 extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
-                               uint64_t jitter_seed, mpic_mesh** out) {
-  if (!out || !lo || !hi || nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6)) {
+                               uint64_t jitter_seed, int mapping, mpic_mesh** out) {
+  if (!out || !lo || !hi || nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6) || mapping < 0 || mapping > 3) {
     mpic::global_error() = "mpic_mesh_brick: bad arguments";
     return MPIC_ERR_ARG;
   }
@@ -268,7 +293,7 @@ extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* 
     m->hi[a] = hi[a];
   }
   BrickBuilder b(*m);
-  if (!b.run(jitter, jitter_seed)) {
+  if (!b.run(jitter, jitter_seed, mapping)) {
     delete m;
     return MPIC_ERR_ARG;
   }

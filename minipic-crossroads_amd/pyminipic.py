@@ -78,7 +78,7 @@ def lib():
         sig = {
             "mpic_config_default": (None, [P]),
             "mpic_last_error": (C.c_char_p, [P]),
-            "mpic_mesh_brick": (i32, [i32, i32, i32, i32, P, P, d, u64, P]),
+            "mpic_mesh_brick": (i32, [i32, i32, i32, i32, P, P, d, u64, i32, P]),
             "mpic_mesh_set_background": (i32, [P, d]),
             "mpic_mesh_describe": (i32, [P, P]),
             "mpic_mesh_free": (None, [P]),
@@ -135,11 +135,12 @@ class Mesh:
     """Host mesh + discretisation built by the library (Mesh/ES/PIC setup)."""
 
     def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0,
-                 background=None):
+                 background=None, mapping=0):
         self._lo = np.array(lo, dtype=np.float64)
         self._hi = np.array(hi, dtype=np.float64)
         h = C.c_void_p()
-        rc = lib().mpic_mesh_brick(nx, ny, nz, split, _p(self._lo), _p(self._hi), jitter, jitter_seed, C.byref(h))
+        rc = lib().mpic_mesh_brick(nx, ny, nz, split, _p(self._lo), _p(self._hi), jitter, jitter_seed, mapping,
+                                   C.byref(h))
         if rc != OK:
             raise MpicError(rc, lib().mpic_last_error(None).decode())
         self.h = h

@@ -92,7 +92,37 @@ struct Mesh {
 
 inline double sq_sum3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-void build_mesh(Mesh& m, double jitter, uint64_t jseed) {
+// Mesh::apply_mapping, mesh.cpp:763-792 (applied to node coordinates after the
+// topology and boundary sets are built, mesh.cpp:587-588).
+void apply_mapping(double& inx, double& iny, double& inz, int mapping) {
+  double outx = inx, outy = iny, outz = inz;
+  switch (mapping) {
+    case 0:
+      return;
+    case 1: {
+      double r = (1.0 - std::fabs(inx)) * (1.0 - std::fabs(iny));
+      outx = inx * std::cos(r) - iny * std::sin(r);
+      outy = inx * std::sin(r) + iny * std::cos(r);
+      outz = inz;
+    } break;
+    case 2:
+      outx = inx * std::sqrt(1 - .5 * iny * iny - .5 * inz * inz + iny * iny * inz * inz / 3.);
+      outy = iny * std::sqrt(1 - .5 * inx * inx - .5 * inz * inz + inx * inx * inz * inz / 3.);
+      outz = inz * std::sqrt(1 - .5 * iny * iny - .5 * inx * inx + iny * iny * inx * inx / 3.);
+      break;
+    case 3:
+      outx = inx * inx;
+      if (inx < 0) outx = -outx;
+      outy = iny;
+      outz = inz;
+      break;
+  }
+  inx = outx;
+  iny = outy;
+  inz = outz;
+}
+
+void build_mesh(Mesh& m, double jitter, uint64_t jseed, int mapping) {
   const int nx = m.nx, ny = m.ny, nz = m.nz;
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   const int tph = m.split;  // tets per hex
@@ -125,6 +155,7 @@ void build_mesh(Mesh& m, double jitter, uint64_t jseed) {
           }
         }
       }
+  for (int64_t g = 0; g < nn; ++g) apply_mapping(m.nodes[g * 3], m.nodes[g * 3 + 1], m.nodes[g * 3 + 2], mapping);
   // Elements, mesh.cpp:196-224 (5-tet) or the Kuhn 6-tet split.
   m.en.assign(ne * 4, 0);
   int64_t cnt = 0;
@@ -520,8 +551,8 @@ void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int orc_get_threads() { return omp_get_max_threads(); }
 
 void* orc_mesh_create(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
-                      uint64_t jseed) {
-  if (nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6)) return nullptr;
+                      uint64_t jseed, int mapping) {
+  if (nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6) || mapping < 0 || mapping > 3) return nullptr;
   Mesh* m = new Mesh();
   m->nx = nx;
   m->ny = ny;
@@ -531,7 +562,7 @@ void* orc_mesh_create(int nx, int ny, int nz, int split, const double* lo, const
     m->lo[a] = lo[a];
     m->hi[a] = hi[a];
   }
-  build_mesh(*m, jitter, jseed);
+  build_mesh(*m, jitter, jseed, mapping);
   if (m->ne < 0) {
     delete m;
     return nullptr;

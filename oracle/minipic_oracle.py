@@ -39,7 +39,7 @@ def lib():
         P = C.c_void_p
         d, i32, i64, u64 = C.c_double, C.c_int, C.c_int64, C.c_uint64
         L.orc_mesh_create.restype = P
-        L.orc_mesh_create.argtypes = [i32, i32, i32, i32, P, P, d, u64]
+        L.orc_mesh_create.argtypes = [i32, i32, i32, i32, P, P, d, u64, i32]
         L.orc_mesh_destroy.argtypes = [P]
         L.orc_mesh_counts.argtypes = [P, P]
         L.orc_mesh_export.argtypes = [P] * 13
@@ -79,13 +79,15 @@ def ptr(a):
 
 class Mesh:
     """Brick mesh: nx*ny*nz hexes on [lo, hi], split 5 (reference, mesh.cpp:83)
-    or 6 (Kuhn) tets per hex, optional seeded interior jitter."""
+    or 6 (Kuhn) tets per hex, optional seeded interior jitter and the
+    reference's node mapping 0-3 (mesh.cpp:763-792)."""
 
-    def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0):
+    def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0,
+                 mapping=0):
         L = lib()
         self.lo = np.array(lo, dtype=np.float64)
         self.hi = np.array(hi, dtype=np.float64)
-        self.h = L.orc_mesh_create(nx, ny, nz, split, ptr(self.lo), ptr(self.hi), jitter, jitter_seed)
+        self.h = L.orc_mesh_create(nx, ny, nz, split, ptr(self.lo), ptr(self.hi), jitter, jitter_seed, mapping)
         if not self.h:
             raise ValueError("bad mesh parameters or degenerate element")
         cnt = np.zeros(4, dtype=np.int64)

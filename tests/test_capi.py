@@ -40,6 +40,9 @@ CASES = [
     dict(nx=3, ny=2, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0),
     dict(nx=4, ny=4, nz=3, split=6, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0),
     dict(nx=5, ny=3, nz=3, split=6, lo=(0, 0, 0), hi=(2, 1, 1.5), jitter=0.2),
+    dict(nx=4, ny=4, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=1),
+    dict(nx=4, ny=3, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=2),
+    dict(nx=4, ny=4, nz=2, split=6, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=3),
 ]
 
 
@@ -47,9 +50,11 @@ CASES = [
 def test_host_setup_matches_oracle(mp, oracle, case):
     c = dict(case)
     jit = c.pop("jitter")
-    o = oracle.Mesh(c["nx"], c["ny"], c["nz"], c["split"], lo=c["lo"], hi=c["hi"], jitter=jit, jitter_seed=11)
+    mapping = c.pop("mapping", 0)
+    o = oracle.Mesh(c["nx"], c["ny"], c["nz"], c["split"], lo=c["lo"], hi=c["hi"], jitter=jit, jitter_seed=11,
+                    mapping=mapping)
     p = mp.Mesh(c["nx"], c["ny"], c["nz"], c["split"], lo=c["lo"], hi=c["hi"], jitter=jit, jitter_seed=11,
-                background=1.75)
+                background=1.75, mapping=mapping)
     pairs = [("nodes", o.nodes), ("elem_nodes", o.elem_nodes), ("elem_nbrs", o.neighbors), ("normals", o.normals),
              ("inv_jacobian", o.jinv), ("det_jacobian", o.detj), ("grad_basis", o.grad), ("wbv", o.wbv),
              ("boundary", o.boundary), ("row_ptr", o.rowptr), ("col_idx", o.col), ("values", o.val),
