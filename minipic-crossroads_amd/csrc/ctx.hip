@@ -48,16 +48,15 @@ struct mpic_ctx {
   PushPartial* partial = nullptr;
   PushPartial* finish = nullptr;   // device [1]
   PushPartial* finish_h = nullptr; // pinned host [1]
-  unsigned long long* bound_d = nullptr;  // [2]
-  unsigned long long* bound_h = nullptr;  // pinned [2]
 
   // fields
   double* phi = nullptr;
   double* rhs = nullptr;
-  fx128* acc = nullptr;  // [ne*4]
+  unsigned long long* acc = nullptr;  // [ne*16] element charge digits
   CGBuffers cg{};
   CGState* cg_h = nullptr;  // pinned
   int last_cg_iters = 0;
+  int rho_hist_cap = 0;
 
   SortBuffers sb{};
   int32_t* start_h = nullptr;  // pinned [1]
@@ -124,15 +123,13 @@ PushArgs push_args(mpic_ctx* c) {
   a.kill = c->cfg.kill_on_wall;
   a.S_charge = c->S;
   a.S_ke = 100;
-  a.acc_tet = c->acc;
+  a.acc = c->acc;
   a.partial = c->partial;
   a.E = c->E;
   a.vr = c->vr;
   a.species = c->species;
   return a;
 }
-
-int nblocks(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
 // Reduce the per-workgroup partials of the last particle launch; synchronises.
 int finish_partials(mpic_ctx* c, int parts, PushPartial* out) {
@@ -145,13 +142,15 @@ int finish_partials(mpic_ctx* c, int parts, PushPartial* out) {
 
 // Charge fixed-point scale from the particles: bound = sum|bg| + n_live * max|w q|.
 int update_charge_scale(mpic_ctx* c) {
-  HIPCHK(c, hipMemsetAsync(c->bound_d, 0, 16, c->stream));
-  HIPCHK(c, launch_charge_bound(c->P[c->cur], c->n, c->species, c->bound_d, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->bound_h, c->bound_d, 16, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  PushPartial r{};
+  if (c->n > 0) {
+    HIPCHK(c, launch_charge_bound(c->P[c->cur], c->n, c->species, c->partial, c->stream));
+    int rc = finish_partials(c, partials_for(c->n), &r);
+    if (rc) return rc;
+  }
   double mx;
-  std::memcpy(&mx, &c->bound_h[0], 8);
-  const double live = (double)c->bound_h[1];
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  const double live = (double)r.crossings;
   const int S = scale_for_bound(c->bg_abs_sum + live * mx);
   if (S != c->S || !c->bgfix) {
     c->S = S;
@@ -171,7 +170,7 @@ int update_charge_scale(mpic_ctx* c) {
 // Particle positions changed outside the fused path: drop a stale deposit.
 int invalidate_deposit(mpic_ctx* c) {
   if (c->acc_deposited) {
-    HIPCHK(c, launch_zero(c->acc, (size_t)c->m.ne * 4 * sizeof(fx128), c->stream));
+    HIPCHK(c, launch_zero(c->acc, (size_t)c->m.ne * 16 * sizeof(unsigned long long), c->stream));
     c->acc_deposited = false;
   }
   return MPIC_OK;
@@ -193,9 +192,7 @@ int free_particles(mpic_ctx* c) {
     }
   if (c->E) (void)hipFree(c->E);
   if (c->vr) (void)hipFree(c->vr);
-  if (c->partial) (void)hipFree(c->partial);
   c->E = c->vr = nullptr;
-  c->partial = nullptr;
   c->cap = c->n = 0;
   return MPIC_OK;
 }
@@ -227,15 +224,23 @@ int do_weight_charge(mpic_ctx* c) {
 int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   c->cg.x = c->phi;
   c->cg.b = c->rhs;
+  if (max_iter + 1 > c->rho_hist_cap) {
+    if (c->cg.rho_hist) (void)hipFree(c->cg.rho_hist);
+    c->cg.rho_hist = nullptr;
+    HIPCHK(c, hipMalloc((void**)&c->cg.rho_hist, sizeof(double) * (size_t)(max_iter + 1)));
+    c->rho_hist_cap = max_iter + 1;
+  }
   HIPCHK(c, launch_cg_init(c->m, c->cg, tol, max_iter, c->stream));
+  // Launch iterations in batches sized from the previous solve; kernels after
+  // convergence return at once.
   int it = 0;
-  int batch = std::max(8, std::min(max_iter, c->last_cg_iters));
+  int batch = std::max(8, std::min(max_iter + 1, c->last_cg_iters + 1));
   while (true) {
-    const int upto = std::min(max_iter, it + batch);
-    for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, c->cg, it, c->stream));
+    const int upto = std::min(max_iter + 1, it + batch);
+    for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, c->cg, it, max_iter, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->cg_h, c->cg.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->cg_h->done || it >= max_iter) break;
+    if (c->cg_h->done || it > max_iter) break;
     batch = 8;
   }
   const CGState& s = *c->cg_h;
@@ -428,23 +433,22 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   if ((rc = dupload(c, &c->species, sp, 48))) return bail(rc);
   // Fields, CG and sort buffers.
   const int nchunks = (nn + BLOCK - 1) / BLOCK;
-  if ((rc = dalloc(c, &c->phi, nn)) || (rc = dalloc(c, &c->rhs, nn)) || (rc = dalloc(c, &c->acc, (size_t)ne * 4)) ||
+  if ((rc = dalloc(c, &c->phi, nn)) || (rc = dalloc(c, &c->rhs, nn)) || (rc = dalloc(c, &c->acc, (size_t)ne * 16)) ||
       (rc = dalloc(c, &c->cg.r, nn)) || (rc = dalloc(c, &c->cg.p0, nn)) || (rc = dalloc(c, &c->cg.p1, nn)) ||
       (rc = dalloc(c, &c->cg.q, nn)) || (rc = dalloc(c, &c->cg.part_bb, nchunks)) ||
       (rc = dalloc(c, &c->cg.part_rr, nchunks)) || (rc = dalloc(c, &c->cg.part_pq, nchunks)) ||
       (rc = dalloc(c, &c->cg.st, 1)) || (rc = dalloc(c, &c->sb.count, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.start, (size_t)ne + 2)) || (rc = dalloc(c, &c->sb.cursor, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.blocksum, (size_t)ne / 2048 + 64)) || (rc = dalloc(c, &c->finish, 1)) ||
-      (rc = dalloc(c, &c->bound_d, 2)))
+      (rc = dalloc(c, &c->partial, MAX_PARTIALS)))
     return bail(rc);
   c->cg.nchunks = nchunks;
   if (hipHostMalloc((void**)&c->cg_h, sizeof(CGState)) != hipSuccess ||
       hipHostMalloc((void**)&c->finish_h, sizeof(PushPartial)) != hipSuccess ||
-      hipHostMalloc((void**)&c->bound_h, 16) != hipSuccess ||
       hipHostMalloc((void**)&c->start_h, sizeof(int32_t)) != hipSuccess)
     return bail(fail(c, MPIC_ERR_HIP, "pinned host allocation failed"));
   if (hipMemset(c->phi, 0, (size_t)nn * 8) != hipSuccess || hipMemset(c->rhs, 0, (size_t)nn * 8) != hipSuccess ||
-      hipMemset(c->acc, 0, (size_t)ne * 4 * sizeof(fx128)) != hipSuccess ||
+      hipMemset(c->acc, 0, (size_t)ne * 16 * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->cg.st, 0, sizeof(CGState)) != hipSuccess)
     return bail(fail(c, MPIC_ERR_HIP, "hipMemset failed"));
   if ((rc = update_charge_scale(c))) return bail(rc);
@@ -458,9 +462,9 @@ void mpic_destroy(mpic_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->cg.rho_hist) (void)hipFree(c->cg.rho_hist);
   if (c->cg_h) (void)hipHostFree(c->cg_h);
   if (c->finish_h) (void)hipHostFree(c->finish_h);
-  if (c->bound_h) (void)hipHostFree(c->bound_h);
   if (c->start_h) (void)hipHostFree(c->start_h);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -493,13 +497,12 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
     P.id = (uint32_t*)(P.elem + cap);
     P.type = (uint8_t*)(P.id + cap);
   }
-  HIPCHK(c, hipMalloc((void**)&c->partial, sizeof(PushPartial) * (size_t)nblocks(cap)));
   c->cap = cap;
   c->n = 0;
   c->cur = 0;
   c->acc_deposited = false;
   c->ke_max_prev = 0;
-  HIPCHK(c, hipMemset(c->acc, 0, (size_t)c->m.ne * 4 * sizeof(fx128)));
+  HIPCHK(c, hipMemset(c->acc, 0, (size_t)c->m.ne * 16 * sizeof(unsigned long long)));
   return MPIC_OK;
 }
 
@@ -638,7 +641,7 @@ int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
   HIPCHK(c, launch_move(a, c->stream));
   tstop(c, T_PUSH);
   PushPartial r{};
-  if (c->n > 0 && (rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+  if (c->n > 0 && (rc = finish_partials(c, partials_for(c->n), &r))) return rc;
   c->ms[T_PUSH] = c->n > 0 ? telapsed(c, T_PUSH) : 0;
   if (st) *st = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
   if (r.killed > 0 && (rc = do_sort(c, nullptr))) return rc;  // cleanup_list, particle_list.cpp:70-73
@@ -655,13 +658,13 @@ int mpic_compute_ke(mpic_ctx* c, double* ke) {
   const DevParticles& P = c->P[c->cur];
   PushPartial r{};
   HIPCHK(c, launch_ke(P, c->n, c->species, 0, 0, c->partial, c->stream));
-  int rc = finish_partials(c, nblocks(c->n), &r);
+  int rc = finish_partials(c, partials_for(c->n), &r);
   if (rc) return rc;
   double mx;
   std::memcpy(&mx, &r.ke_max_bits, 8);
-  const int S = scale_for_bound((double)c->n * mx);
+  const int S = scale_for_bound((double)r.crossings * mx);
   HIPCHK(c, launch_ke(P, c->n, c->species, S, 1, c->partial, c->stream));
-  if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+  if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
   *ke = from_fixed(fx128{r.ke_lo, r.ke_hi}, S);
   c->ke_max_prev = mx;
   return MPIC_OK;
@@ -689,7 +692,7 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   if (!(c->ke_max_prev > 0) && c->n > 0) {
     PushPartial r{};
     HIPCHK(c, launch_ke(c->P[c->cur], c->n, c->species, 0, 0, c->partial, c->stream));
-    if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+    if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
     std::memcpy(&c->ke_max_prev, &r.ke_max_bits, 8);
   }
   PushArgs a = push_args(c);
@@ -701,7 +704,7 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   tstop(c, T_PUSH_KERNEL);
   PushPartial r{};
   if (c->n > 0) {
-    if ((rc = finish_partials(c, nblocks(c->n), &r))) return rc;
+    if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
   }
   tstop(c, T_PUSH);
   c->acc_deposited = true;
@@ -714,7 +717,7 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     // The provisional scale could have overflowed: sum again at a safe scale.
     PushPartial k{};
     HIPCHK(c, launch_ke(c->P[c->cur], c->n, c->species, S_safe, 1, c->partial, c->stream));
-    if ((rc = finish_partials(c, nblocks(c->n), &k))) return rc;
+    if ((rc = finish_partials(c, partials_for(c->n), &k))) return rc;
     st.kinetic_energy = from_fixed(fx128{k.ke_lo, k.ke_hi}, S_safe);
   } else {
     st.kinetic_energy = from_fixed(fx128{r.ke_lo, r.ke_hi}, a.S_ke);

@@ -2,12 +2,13 @@
 // on the device with no host round trip per iteration.
 //
 // One iteration = two launches:
-//   k1: p = beta*p_old + r (p = r at iteration 0) fused into q = A p: every
-//       row recomputes the p entries of its columns, so p is never re-read after
-//       the update; partial sums of p.q.
-//   k2: x += alpha p, r += (-alpha) q, partial sums of r.r.
-// The last workgroup of each launch (found with an atomic counter) reduces the
-// partials and updates the scalars (alpha; rho, beta, convergence) in CGState.
+//   k1: convergence test of the previous residual; p = beta*p_old + r (p = r at
+//       iteration 0) fused into q = A p -- every row recomputes the p entries of
+//       its columns, so p is never re-read after the update; partials of p.q.
+//   k2: alpha = rho / p.q; x += alpha p, r += (-alpha) q; partials of r.r.
+// Each launch reduces the previous launch's partials redundantly in every
+// workgroup (they are visible at the kernel boundary), so no atomics, fences or
+// extra reduction launches are needed; workgroup 0 records the scalars.
 //
 // Dot products use one fixed tree: chunk = 256 rows = one workgroup, item t held
 // by thread t as (0 + term); xor butterfly over the 64 lanes of a wave
@@ -37,38 +38,32 @@ __device__ __forceinline__ double block_tree(double v) {
   return (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
-__device__ __forceinline__ double reduce_partials(const double* part, int P) {
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int P) {
   double acc = 0;
-  for (int i = threadIdx.x; i < P; i += BLOCK) acc += __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < P; i += BLOCK) acc += part[i];
   return block_tree(acc);
 }
 
-// Publishes this workgroup's partial and returns true in the last workgroup to finish.
-__device__ __forceinline__ bool publish_and_check_last(double* part, double val, unsigned int* counter) {
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(part + blockIdx.x, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-// Sequential ELL row sum: acc = 0; acc += val_k * x(col_k) in CSR order.
+// Sequential ELL row sum: acc = 0; acc += val_k * x(col_k) in CSR order.  Loads
+// are issued four entries at a time; the additions keep their order.
 template <class F>
 __device__ __forceinline__ double ell_row(const DevMesh& m, int row, F xval) {
   const int slice = row >> 6, lane = row & 63;
   const int len = m.ell_len[slice];
-  const int64_t base = m.ell_off[slice] + lane;
+  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
+  const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
   double acc = 0;
-  for (int k = 0; k < len; ++k) {
-    const int c = m.ell_col[base + (int64_t)k * 64];
-    const double v = m.ell_val[base + (int64_t)k * 64];
-    acc += v * xval(c);
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const int c0 = cp[(k + 0) * 64], c1 = cp[(k + 1) * 64], c2 = cp[(k + 2) * 64], c3 = cp[(k + 3) * 64];
+    const double v0 = vp[(k + 0) * 64], v1 = vp[(k + 1) * 64], v2 = vp[(k + 2) * 64], v3 = vp[(k + 3) * 64];
+    const double x0 = xval(c0), x1 = xval(c1), x2 = xval(c2), x3 = xval(c3);
+    acc += v0 * x0;
+    acc += v1 * x1;
+    acc += v2 * x2;
+    acc += v3 * x3;
   }
+  for (; k < len; ++k) acc += vp[k * 64] * xval(cp[k * 64]);
   return acc;
 }
 
@@ -76,17 +71,18 @@ struct CGArgs {
   DevMesh m;
   double *x, *b, *r, *pold, *pnew, *q;
   double *part_bb, *part_rr, *part_pq;
+  double* rho_hist;  // [max_iter+1] rho of each iteration
   CGState* st;
   int it;
 };
 
-// r = b - A x; rho0 = b.b; rho = r.r (CG_solve.hpp:58-72).
+// r = b - A x; partials of b.b and r.r (CG_solve.hpp:58-66).
 __global__ __launch_bounds__(BLOCK) void k_cg_init(CGArgs a) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   double tb = 0, tr = 0;
   if (i < a.m.nn) {
     const double bi = a.b[i];
-    const double* x = a.x;
+    const double* __restrict__ x = a.x;
     const double ax = ell_row(a.m, i, [&](int c) { return x[c]; });
     const double ri = bi - ax;
     a.r[i] = ri;
@@ -95,46 +91,49 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init(CGArgs a) {
   }
   const double sb = block_tree(tb);
   const double sr = block_tree(tr);
-  __shared__ int last;
   if (threadIdx.x == 0) {
-    __hip_atomic_store(a.part_bb + blockIdx.x, sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(a.part_rr + blockIdx.x, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned int prev = __hip_atomic_fetch_add(&a.st->cnt_init, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const double rho0 = reduce_partials(a.part_bb, gridDim.x);
-  const double rho = reduce_partials(a.part_rr, gridDim.x);
-  if (threadIdx.x == 0) {
-    CGState* st = a.st;
-    st->cnt_init = 0;
-    st->rho0 = rho0;
-    st->rho = rho;
-    st->iters = 0;
-    st->done = (rho == 0) ? 1 : 0;
-    st->converged = (rho == 0) ? 1 : 0;
-    __threadfence();
+    a.part_bb[blockIdx.x] = sb;
+    a.part_rr[blockIdx.x] = sr;
   }
 }
 
-// p update fused into q = A p; partial p.q; alpha = rho / p.q (CG_solve.hpp:77-88).
+// Convergence test (CG_solve.hpp:71-72,103-107; max_iter :73,113), then
+// p = beta p_old + r fused into q = A p; partials of p.q (CG_solve.hpp:77-87).
 __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
-  if (a.st->done) return;
+  CGState* st = a.st;
+  if (st->done) return;
+  const int it = a.it;
+  const double rho = reduce_partials(a.part_rr, gridDim.x);
+  const double rho0 = it == 0 ? reduce_partials(a.part_bb, gridDim.x) : st->rho0;
+  bool stop = false, conv = false;
+  if (it == 0) {
+    if (rho == 0) stop = conv = true;
+  } else if (rho / rho0 < st->tol2) {
+    stop = conv = true;
+  }
+  if (!stop && it >= st->max_iter) stop = true;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rho = rho;
+    if (it == 0) st->rho0 = rho0;
+    a.rho_hist[it] = rho;
+    if (stop) {
+      st->done = 1;
+      st->converged = conv ? 1 : 0;
+      st->iters = it;
+    }
+  }
+  if (stop) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const bool first = (a.it == 0);
-  const double beta = a.st->beta;
   double t = 0;
   if (i < a.m.nn) {
-    const double* r = a.r;
-    const double* po = a.pold;
+    const double* __restrict__ r = a.r;
+    const double* __restrict__ po = a.pold;
     double pi, qi;
-    if (first) {
+    if (it == 0) {
       pi = r[i];
       qi = ell_row(a.m, i, [&](int c) { return r[c]; });
     } else {
+      const double beta = rho / a.rho_hist[it - 1];
       pi = beta * po[i] + r[i];
       qi = ell_row(a.m, i, [&](int c) { return beta * po[c] + r[c]; });
     }
@@ -143,24 +142,15 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
     t += pi * qi;
   }
   const double s = block_tree(t);
-  if (!publish_and_check_last(a.part_pq, s, &a.st->cnt_k1)) return;
-  const double pq = reduce_partials(a.part_pq, gridDim.x);
-  if (threadIdx.x == 0) {
-    CGState* st = a.st;
-    st->cnt_k1 = 0;
-    double alpha = pq;
-    alpha = st->rho / alpha;
-    st->alpha = alpha;
-    __threadfence();
-  }
+  if (threadIdx.x == 0) a.part_pq[blockIdx.x] = s;
 }
 
-// x += alpha p; r += (-alpha) q; rho = r.r; convergence test rho/rho0 < tol^2
-// (CG_solve.hpp:91-107); beta = rho/rho_old for the next iteration.
+// alpha = rho / p.q; x += alpha p; r += (-alpha) q; partials of r.r (CG_solve.hpp:87-100).
 __global__ __launch_bounds__(BLOCK) void k_cg_k2(CGArgs a) {
   if (a.st->done) return;
+  double alpha = reduce_partials(a.part_pq, gridDim.x);
+  alpha = a.rho_hist[a.it] / alpha;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const double alpha = a.st->alpha;
   double t = 0;
   if (i < a.m.nn) {
     a.x[i] = a.x[i] + alpha * a.pnew[i];
@@ -169,24 +159,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k2(CGArgs a) {
     t += ri * ri;
   }
   const double s = block_tree(t);
-  if (!publish_and_check_last(a.part_rr, s, &a.st->cnt_k2)) return;
-  const double rho = reduce_partials(a.part_rr, gridDim.x);
-  if (threadIdx.x == 0) {
-    CGState* st = a.st;
-    st->cnt_k2 = 0;
-    st->rho_old = st->rho;
-    st->rho = rho;
-    st->iters = a.it + 1;
-    if (rho / st->rho0 < st->tol2) {
-      st->done = 1;
-      st->converged = 1;
-    } else if (a.it + 1 >= st->max_iter) {
-      st->done = 1;
-      st->converged = 0;
-    }
-    st->beta = rho / st->rho_old;
-    __threadfence();
-  }
+  if (threadIdx.x == 0) a.part_rr[blockIdx.x] = s;
 }
 
 __global__ void k_cg_setup(CGState* st, double tol2, int max_iter) {
@@ -208,6 +181,7 @@ CGArgs make_args(const DevMesh& m, CGBuffers& c, int it) {
   a.part_bb = c.part_bb;
   a.part_rr = c.part_rr;
   a.part_pq = c.part_pq;
+  a.rho_hist = c.rho_hist;
   a.st = c.st;
   a.it = it;
   return a;
@@ -221,10 +195,10 @@ hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_it
   return hipGetLastError();
 }
 
-hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, hipStream_t s) {
+hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s) {
   const CGArgs a = make_args(m, c, it);
   hipLaunchKernelGGL(k_cg_k1, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_cg_k2, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
+  if (it < max_iter) hipLaunchKernelGGL(k_cg_k2, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

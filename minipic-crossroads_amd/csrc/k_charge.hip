@@ -7,41 +7,25 @@ namespace mpic {
 namespace {
 
 // rhs[i] = background[i] + sum of the element slots touching node i, exact in
-// fixed point; boundary rows are zeroed (PIC.cpp:155-158).  Each slot belongs to
-// exactly one node, so the slot is cleared here for the next deposit.
-__global__ __launch_bounds__(BLOCK) void k_node_charge(DevMesh m, fx128* __restrict__ acc, int S,
+// fixed point; boundary rows are zeroed (PIC.cpp:155-158).  Each slot (element
+// e, local node k: 4 digit words at acc[(e*4+k)*4]) belongs to exactly one node,
+// so the slot is cleared here for the next deposit.
+__global__ __launch_bounds__(BLOCK) void k_node_charge(DevMesh m, unsigned long long* __restrict__ acc, int S,
                                                        double* __restrict__ rhs) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= m.nn) return;
   fx128 s = m.bgfix ? m.bgfix[i] : fx128{0, 0};
   const int b = m.adj_ptr[i], e = m.adj_ptr[i + 1];
   for (int k = b; k < e; ++k) {
-    const int slot = m.adj[k];
-    s = fx_add(s, acc[slot]);
-    acc[slot] = fx128{0, 0};
+    unsigned long long* w = acc + (size_t)m.adj[k] * 4;  // slot e*4+node: 4 digit words
+    const ulonglong2 w01 = *reinterpret_cast<const ulonglong2*>(w);
+    const ulonglong2 w23 = *reinterpret_cast<const ulonglong2*>(w + 2);
+    const unsigned long long ws[4] = {w01.x, w01.y, w23.x, w23.y};
+    s = fx_add(s, dep_slot_value(ws));
+    *reinterpret_cast<ulonglong2*>(w) = ulonglong2{0, 0};
+    *reinterpret_cast<ulonglong2*>(w + 2) = ulonglong2{0, 0};
   }
   rhs[i] = m.bnd[i] ? 0.0 : from_fixed(s, S);
-}
-
-// max |w*q| over live particles (as ordered double bits) and the live count.
-__global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t n, const double* sp,
-                                                        unsigned long long* out2) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  unsigned long long mx = 0, live = 0;
-  if (p < n && P.elem[p] >= 0) {
-    mx = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
-    live = 1;
-  }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const unsigned long long o = __shfl_xor(mx, off);
-    mx = o > mx ? o : mx;
-    live += __shfl_xor(live, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (mx) atomicMax(out2, mx);
-    if (live) atomicAdd(out2 + 1, live);
-  }
 }
 
 __global__ void k_zero(unsigned long long* p, size_t n) {
@@ -50,16 +34,8 @@ __global__ void k_zero(unsigned long long* p, size_t n) {
 
 }  // namespace
 
-hipError_t launch_node_charge(const DevMesh& m, fx128* acc_tet, int S, double* rhs, hipStream_t s) {
-  hipLaunchKernelGGL(k_node_charge, dim3((m.nn + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, m, acc_tet, S, rhs);
-  return hipGetLastError();
-}
-
-hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* species, unsigned long long* out2,
-                               hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_charge_bound, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, P, n, species,
-                     out2);
+hipError_t launch_node_charge(const DevMesh& m, unsigned long long* acc, int S, double* rhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_node_charge, dim3((m.nn + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, m, acc, S, rhs);
   return hipGetLastError();
 }
 

@@ -174,12 +174,11 @@ __device__ __forceinline__ double ke_term(double w, double mass, const double V[
 }
 
 // Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x].
-__device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, double kemax, const WalkCount& wc,
-                                              int killed) {
+__device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsigned long long kmax, const WalkCount& wc,
+                                              long long killed) {
   __shared__ PushPartial sh[BLOCK / 64];
   unsigned long long klo = ke.lo;
   long long khi = ke.hi;
-  unsigned long long kmax = __double_as_longlong(kemax);
   long long c0 = wc.crossings, c1 = killed, c2 = wc.capped, c3 = wc.nonpos, c4 = wc.reflections;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -202,9 +201,9 @@ __device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, double
   if (threadIdx.x == 0) {
     PushPartial r = sh[0];
     for (int w = 1; w < BLOCK / 64; ++w) {
-      fx128 s = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
-      r.ke_lo = s.lo;
-      r.ke_hi = s.hi;
+      fx128 t = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
+      r.ke_lo = t.lo;
+      r.ke_hi = t.hi;
       r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
       r.crossings += sh[w].crossings;
       r.killed += sh[w].killed;
@@ -223,60 +222,86 @@ struct KArgs {
   const double* phi;
   double dt, scale;
   int kill, S_charge, S_ke;
-  fx128* acc_tet;
+  unsigned long long* acc;
   PushPartial* partial;
   double* E;
   double* vr;
   const double* sp;  // [48]: q[16], m[16], q/m[16]
 };
 
+// Each workgroup handles one contiguous range of particles in chunks of BLOCK
+// (a wavefront covers 64 consecutive particles); element charges of the new
+// positions are accumulated in an LDS table and flushed every FLUSH_CHUNKS.
+constexpr int FLUSH_CHUNKS = 4;
+
+__device__ __forceinline__ void block_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+  const int64_t per = (chunks + gridDim.x - 1) / gridDim.x;
+  *lo = (int64_t)blockIdx.x * per * BLOCK;
+  const int64_t h = *lo + per * BLOCK;
+  *hi = h < n ? h : n;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  __shared__ DepTable table;
+  dep_table_clear(table);
   const int64_t cap = a.P.cap;
-  int enew = -1, killed = 0;
-  double w = 0, V[3] = {0, 0, 0};
-  int t = 0;
   WalkCount wc{0, 0, 0, 0};
-  if (p < a.n) {
-    int e = a.P.elem[p];
-    if (e >= 0) {
-      double X[3], R[3];
+  long long killed = 0;
+  fx128 ke{0, 0};
+  unsigned long long kmax = 0;
+  int64_t lo, hi;
+  block_range(a.n, &lo, &hi);
+  __syncthreads();
+  int chunk = 0;
+  for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
+    const int64_t p = base + threadIdx.x;
+    int enew = -1;
+    double w = 0, V[3] = {0, 0, 0};
+    int t = 0;
+    if (p < hi) {
+      int e = a.P.elem[p];
+      if (e >= 0) {
+        double X[3], R[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        X[d] = a.P.x[d * cap + p];
-        R[d] = a.P.xr[d * cap + p];
-        V[d] = a.P.v[d * cap + p];
-      }
-      t = a.P.type[p];
-      w = a.P.w[p];
-      double E[3];
-      gather_E(a.m, a.phi, e, R, E);
-      double g[12];
-      load12(a.m.geo, e, g);
-      double VR[3];
-      kick(g, V, E, a.sp[32 + t], a.dt, a.scale, VR);
-      const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+        for (int d = 0; d < 3; ++d) {
+          X[d] = a.P.x[d * cap + p];
+          R[d] = a.P.xr[d * cap + p];
+          V[d] = a.P.v[d * cap + p];
+        }
+        t = a.P.type[p];
+        w = a.P.w[p];
+        double E[3];
+        gather_E(a.m, a.phi, e, R, E);
+        double g[12];
+        load12(a.m.geo, e, g);
+        double VR[3];
+        kick(g, V, E, a.sp[32 + t], a.dt, a.scale, VR);
+        const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        a.P.x[d * cap + p] = X[d];
-        a.P.xr[d * cap + p] = R[d];
-        a.P.v[d * cap + p] = V[d];
+        for (int d = 0; d < 3; ++d) {
+          a.P.x[d * cap + p] = X[d];
+          a.P.xr[d * cap + p] = R[d];
+          a.P.v[d * cap + p] = V[d];
+        }
+        enew = alive ? e : -1;
+        killed += alive ? 0 : 1;
+        a.P.elem[p] = enew;
       }
-      enew = alive ? e : -1;
-      killed = alive ? 0 : 1;
-      a.P.elem[p] = enew;
     }
+    if (enew >= 0) {
+      fx128 c[4];
+      particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
+      dep_table_add(table, a.acc, enew, c);
+      const double term = ke_term(w, a.sp[16 + t], V);
+      ke = fx_add(ke, to_fixed(term, a.S_ke));
+      const unsigned long long tb = __double_as_longlong(term);
+      kmax = tb > kmax ? tb : kmax;
+    }
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, a.acc);
   }
-  fx128 c[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  if (enew >= 0) particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
-  deposit_segmented(enew, c, a.acc_tet);
-  double term = 0;
-  fx128 k{0, 0};
-  if (enew >= 0) {
-    term = ke_term(w, a.sp[16 + t], V);
-    k = to_fixed(term, a.S_ke);
-  }
-  block_partial(a.partial, k, term, wc, killed);
+  dep_table_flush(table, a.acc);
+  block_partial(a.partial, ke, kmax, wc, killed);
 }
 
 // weight_Efield: E of every live particle (PIC.cpp:186-189).
@@ -315,62 +340,93 @@ __global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
 
 // MoveParticle: the tet walk from the stored v and v_ref (no deposit).
 __global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const int64_t cap = a.P.cap;
   WalkCount wc{0, 0, 0, 0};
-  int killed = 0;
-  if (p < a.n) {
+  long long killed = 0;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < a.n; base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t p = base + threadIdx.x;
+    if (p >= a.n) continue;
     int e = a.P.elem[p];
-    if (e >= 0) {
-      double X[3], R[3], V[3], VR[3];
-      for (int d = 0; d < 3; ++d) {
-        X[d] = a.P.x[d * cap + p];
-        R[d] = a.P.xr[d * cap + p];
-        V[d] = a.P.v[d * cap + p];
-        VR[d] = a.vr[d * cap + p];
-      }
-      const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
-      for (int d = 0; d < 3; ++d) {
-        a.P.x[d * cap + p] = X[d];
-        a.P.xr[d * cap + p] = R[d];
-        a.P.v[d * cap + p] = V[d];
-      }
-      a.P.elem[p] = alive ? e : -1;
-      killed = alive ? 0 : 1;
+    if (e < 0) continue;
+    double X[3], R[3], V[3], VR[3];
+    for (int d = 0; d < 3; ++d) {
+      X[d] = a.P.x[d * cap + p];
+      R[d] = a.P.xr[d * cap + p];
+      V[d] = a.P.v[d * cap + p];
+      VR[d] = a.vr[d * cap + p];
     }
+    const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+    for (int d = 0; d < 3; ++d) {
+      a.P.x[d * cap + p] = X[d];
+      a.P.xr[d * cap + p] = R[d];
+      a.P.v[d * cap + p] = V[d];
+    }
+    a.P.elem[p] = alive ? e : -1;
+    killed += alive ? 0 : 1;
   }
-  block_partial(a.partial, fx128{0, 0}, 0.0, wc, killed);
+  block_partial(a.partial, fx128{0, 0}, 0ull, wc, killed);
 }
 
 // Deposit of the current positions (weight_charge, PIC.cpp:139-160).
 __global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, int64_t n, const double* sp, int S,
-                                                   fx128* acc_tet) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  int e = -1;
-  fx128 c[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  if (p < n) {
-    e = P.elem[p];
-    if (e >= 0) particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
+                                                   unsigned long long* acc) {
+  __shared__ DepTable table;
+  dep_table_clear(table);
+  int64_t lo, hi;
+  block_range(n, &lo, &hi);
+  __syncthreads();
+  int chunk = 0;
+  for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
+    const int64_t p = base + threadIdx.x;
+    if (p < hi) {
+      const int e = P.elem[p];
+      if (e >= 0) {
+        fx128 c[4];
+        particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
+        dep_table_add(table, acc, e, c);
+      }
+    }
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, acc);
   }
-  deposit_segmented(e, c, acc_tet);
+  dep_table_flush(table, acc);
 }
 
-// KE: max term pass and sum pass (PIC.cpp:191-200).
+// KE terms (PIC.cpp:191-200): the per-workgroup max term, the live count (in
+// the crossings slot) and, when want_sum, the fixed-point sum at scale S.
 __global__ __launch_bounds__(BLOCK) void k_ke(DevParticles P, int64_t n, const double* sp, int S, int want_sum,
                                               PushPartial* partial) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  double term = 0;
-  fx128 k{0, 0};
-  if (p < n && P.elem[p] >= 0) {
+  fx128 ke{0, 0};
+  unsigned long long kmax = 0;
+  WalkCount live{0, 0, 0, 0};
+  for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
+    if (P.elem[p] < 0) continue;
     double V[3];
     for (int d = 0; d < 3; ++d) V[d] = P.v[d * P.cap + p];
-    term = ke_term(P.w[p], sp[16 + P.type[p]], V);
-    if (want_sum) k = to_fixed(term, S);
+    const double term = ke_term(P.w[p], sp[16 + P.type[p]], V);
+    if (want_sum) ke = fx_add(ke, to_fixed(term, S));
+    const unsigned long long tb = __double_as_longlong(term);
+    kmax = tb > kmax ? tb : kmax;
+    live.crossings++;
   }
-  block_partial(partial, k, term, WalkCount{0, 0, 0, 0}, 0);
+  block_partial(partial, ke, kmax, live, 0);
 }
 
-// Final reduction of per-workgroup partials (one workgroup, integer sums).
+// Charge bound for the fixed-point scale: max |w q| (in the max slot) and the
+// live count (in the crossings slot).
+__global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t n, const double* sp,
+                                                        PushPartial* partial) {
+  unsigned long long qmax = 0;
+  WalkCount live{0, 0, 0, 0};
+  for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
+    if (P.elem[p] < 0) continue;
+    const unsigned long long qb = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
+    qmax = qb > qmax ? qb : qmax;
+    live.crossings++;
+  }
+  block_partial(partial, fx128{0, 0}, qmax, live, 0);
+}
+
+// Final reduction of per-workgroup partials (one workgroup, integer sums and maxima).
 __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, int nparts, PushPartial* out) {
   fx128 ke{0, 0};
   unsigned long long mx = 0;
@@ -406,17 +462,30 @@ __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, 
 }
 
 KArgs kargs(const PushArgs& a) {
-  return KArgs{a.m, a.P, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge, a.S_ke, a.acc_tet, a.partial, a.E, a.vr,
+  return KArgs{a.m, a.P, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge, a.S_ke, a.acc, a.partial, a.E, a.vr,
                a.species};
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+inline unsigned grid_stride(int64_t n) {
+  const int64_t g = (n + BLOCK - 1) / BLOCK;
+  return (unsigned)(g < MAX_PARTIALS ? g : MAX_PARTIALS);
+}
 
 }  // namespace
 
+int partials_for(int64_t n) { return (int)grid_stride(n); }
+
+hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* species, PushPartial* partial,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_charge_bound, dim3(grid_stride(n)), dim3(BLOCK), 0, s, P, n, species, partial);
+  return hipGetLastError();
+}
+
 hipError_t launch_push_fused(const PushArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_push_fused, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  hipLaunchKernelGGL(k_push_fused, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, kargs(a));
   return hipGetLastError();
 }
 hipError_t launch_gather(const PushArgs& a, hipStream_t s) {
@@ -431,7 +500,7 @@ hipError_t launch_accel(const PushArgs& a, hipStream_t s) {
 }
 hipError_t launch_move(const PushArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_move, dim3(grid_for(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  hipLaunchKernelGGL(k_move, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, kargs(a));
   return hipGetLastError();
 }
 hipError_t launch_push_finish(const PushPartial* part, int nparts, PushPartial* out, hipStream_t s) {
@@ -439,15 +508,15 @@ hipError_t launch_push_finish(const PushPartial* part, int nparts, PushPartial* 
   return hipGetLastError();
 }
 hipError_t launch_deposit(const DevMesh& m, const DevParticles& P, int64_t n, const double* species, int S,
-                          fx128* acc_tet, hipStream_t s) {
+                          unsigned long long* acc, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_deposit, dim3(grid_for(n)), dim3(BLOCK), 0, s, m, P, n, species, S, acc_tet);
+  hipLaunchKernelGGL(k_deposit, dim3(grid_stride(n)), dim3(BLOCK), 0, s, m, P, n, species, S, acc);
   return hipGetLastError();
 }
 hipError_t launch_ke(const DevParticles& P, int64_t n, const double* species, int S, int want_sum,
                      PushPartial* partial, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ke, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, n, species, S, want_sum, partial);
+  hipLaunchKernelGGL(k_ke, dim3(grid_stride(n)), dim3(BLOCK), 0, s, P, n, species, S, want_sum, partial);
   return hipGetLastError();
 }
 

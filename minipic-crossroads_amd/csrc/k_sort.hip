@@ -6,8 +6,9 @@
 //   3. scatter every live particle to start[e] + rank; dead particles
 //      (elem < 0) are dropped, which also compacts the list
 //      (DeleteParticles, particle_list.cpp:148-191).
-// As in the reference, the order inside one element is not specified; the
-// identity key (id) travels with the particle.
+// As in the reference, the order inside one element is not specified (it
+// depends on the order of the per-wavefront atomics); the identity key (id)
+// travels with the particle.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
 
@@ -17,29 +18,15 @@ namespace {
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 
-// Per-lane run info for contiguous equal keys in a wavefront.
-struct Run {
-  bool head;
-  int head_lane;
-  int len;  // valid in the head lane
-};
-__device__ __forceinline__ Run wave_runs(int key) {
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1);
-  const bool head = (lane == 0) || (prev != key);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int head_lane = 63 - __clzll(heads & le);
-  const unsigned long long after = heads & ~le;
-  const int next = after ? (__ffsll((long long)after) - 1) : 64;
-  return Run{head, head_lane, next - lane};
-}
-
+// Lanes of a wavefront with the same element are grouped (wave_group_leader);
+// the leader adds the group size with one atomic, so every atomic instruction
+// touches distinct addresses.
 __global__ __launch_bounds__(BLOCK) void k_count(const int32_t* __restrict__ elem, int64_t n, int32_t* count) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const int e = p < n ? elem[p] : -1;
-  const Run r = wave_runs(e);
-  if (r.head && e >= 0) atomicAdd(count + e, r.len);
+  unsigned long long grp;
+  const int leader = wave_group_leader(e, &grp);
+  if (leader == (int)(threadIdx.x & 63)) atomicAdd(count + e, (int)__popcll(grp));
 }
 
 // Exclusive scan, pass 1: tile totals.
@@ -120,13 +107,16 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
 __global__ __launch_bounds__(BLOCK) void k_scatter(DevParticles src, DevParticles dst, int64_t n,
                                                    int32_t* __restrict__ cursor) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int e = p < n ? src.elem[p] : -1;
-  const Run r = wave_runs(e);
+  unsigned long long grp;
+  const int leader = wave_group_leader(e, &grp);
   int base = 0;
-  if (r.head && e >= 0) base = atomicAdd(cursor + e, r.len);
-  base = __shfl(base, r.head_lane);
+  if (leader == lane) base = atomicAdd(cursor + e, (int)__popcll(grp));
+  base = __shfl(base, leader < 0 ? lane : leader);
   if (e < 0) return;
-  const int64_t q = base + ((threadIdx.x & 63) - r.head_lane);
+  const unsigned long long below = (lane == 0) ? 0ull : (grp & ((1ull << lane) - 1ull));
+  const int64_t q = base + __popcll(below);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     dst.x[d * dst.cap + q] = src.x[d * src.cap + p];

@@ -51,34 +51,137 @@ __device__ __forceinline__ void atomic_add_fx(fx128* dst, fx128 v) {
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wavefront-segmented charge deposit.  Lanes holding the same element in a
-// contiguous run (particles sorted by element) are summed with shuffles; the
-// first lane of each run adds the run's 4 node charges to the element's
-// accumulator slots acc[key*4 + i].  key < 0 lanes contribute nothing.
-__device__ __forceinline__ void deposit_segmented(int key, fx128 c[4], fx128* __restrict__ acc) {
+// Groups the lanes of a wavefront by key.  Returns the lane index of the
+// group's leader (its lowest lane) for lanes with key >= 0, -1 otherwise; the
+// loop runs once per distinct key in the wavefront.
+__device__ __forceinline__ int wave_group_leader(int key, unsigned long long* group_mask) {
   const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1);
-  const bool head = (lane == 0) || (prev != key);
-  const unsigned long long heads = __ballot(head);
-  if (heads != ~0ull) {
-    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const int run = __popcll(heads & le);
+  unsigned long long pending = __ballot(key >= 0);
+  int leader = -1;
+  unsigned long long mine = 0;
+  while (pending) {
+    const int l = __ffsll((long long)pending) - 1;
+    const int k = __shfl(key, l);
+    const unsigned long long grp = __ballot(key == k) & pending;
+    if ((grp >> lane) & 1ull) {
+      leader = l;
+      mine = grp;
+    }
+    pending &= ~grp;
+  }
+  *group_mask = mine;
+  return leader;
+}
+
+// 128-bit add into LDS (64-bit LDS atomics, carry folded into the high word).
+__device__ __forceinline__ void lds_add_fx(fx128* dst, fx128 v) {
+  const unsigned long long old = atomicAdd(&dst->lo, v.lo);
+  const unsigned long long carry = (old + v.lo) < old ? 1ull : 0ull;
+  const unsigned long long h = (unsigned long long)v.hi + carry;
+  if (h != 0ull) atomicAdd(reinterpret_cast<unsigned long long*>(&dst->hi), h);
+}
+
+// ---------------------------------------------------------------------------
+// Block-level charge accumulation.
+//
+// Element charges are accumulated exactly (128-bit) in an LDS hash table keyed
+// by element, over a contiguous run of particles, and flushed to the global
+// accumulator with non-returning atomics.  The global accumulator holds each
+// (element, node) value as four 64-bit words of 32-bit digits: bits 0-31,
+// 32-63, 64-95 of the 128-bit two's-complement value (unsigned) and bits 96-127
+// (signed).  A word receives at most one digit per flush and there are fewer
+// flushes per slot than particles (< 2^31), so no word can overflow and no carry
+// is needed; the exact value is d0 + d1*2^32 + d2*2^64 + d3*2^96 (mod 2^128).
+// The 16 words of an element (4 nodes x 4 digits) are one 128-byte line, and a
+// flush writes them with 16 consecutive lanes.
+// ---------------------------------------------------------------------------
+constexpr int DEP_TABLE = 512;  // LDS hash entries per workgroup (power of two)
+constexpr int DEP_EMPTY = -1;
+
+struct DepTable {
+  int key[DEP_TABLE];
+  fx128 val[DEP_TABLE][4];
+};
+
+__device__ __forceinline__ void dep_table_clear(DepTable& t) {
+  for (int k = threadIdx.x; k < DEP_TABLE; k += BLOCK) {
+    t.key[k] = DEP_EMPTY;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int orun = __shfl_down(run, off);
-      const bool take = (lane + off < 64) && (orun == run);
+    for (int i = 0; i < 4; ++i) t.val[k][i] = fx128{0, 0};
+  }
+}
+
+// Digit d (0..3) of a 128-bit value as a 64-bit word (see above).
+__device__ __forceinline__ unsigned long long fx_digit(fx128 v, int d) {
+  switch (d) {
+    case 0: return v.lo & 0xffffffffull;
+    case 1: return v.lo >> 32;
+    case 2: return (unsigned long long)v.hi & 0xffffffffull;
+    default: return (unsigned long long)(v.hi >> 32);  // arithmetic shift: signed top digit
+  }
+}
+
+// Direct global add of one particle's 4 node charges (table overflow path).
+__device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ acc, int key, const fx128 c[4]) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const unsigned long long olo = __shfl_down(c[i].lo, off);
-        const long long ohi = __shfl_down(c[i].hi, off);
-        if (take) c[i] = fx_add(c[i], fx128{olo, ohi});
-      }
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const unsigned long long w = fx_digit(c[i], d);
+      if (w) __hip_atomic_fetch_add(acc + (size_t)key * 16 + i * 4 + d, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Adds one particle's node charges to the table entry of element key (key < 0:
+// nothing).  Uses LDS atomics; falls back to global atomics if the table is full.
+__device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* __restrict__ acc, int key,
+                                              const fx128 c[4]) {
+  if (key < 0) return;
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);  // 9 = log2(DEP_TABLE)
+  for (int probe = 0; probe < DEP_TABLE; ++probe) {
+    int k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == DEP_EMPTY) {
+      int expected = DEP_EMPTY;
+      __hip_atomic_compare_exchange_strong(&t.key[h], &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      k = expected == DEP_EMPTY ? key : expected;
+    }
+    if (k == key) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) lds_add_fx(&t.val[h][i], c[i]);
+      return;
+    }
+    h = (h + 1) & (DEP_TABLE - 1);
+  }
+  dep_global_add(acc, key, c);
+}
+
+// Flushes every used entry with coalesced non-returning atomics and clears the
+// table.  Must be called by the whole workgroup.
+__device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long* __restrict__ acc) {
+  __syncthreads();
+  const int sub = threadIdx.x & 15;  // node = sub/4, digit = sub%4
+  for (int e = threadIdx.x >> 4; e < DEP_TABLE; e += BLOCK / 16) {
+    const int key = t.key[e];
+    if (key != DEP_EMPTY) {
+      const unsigned long long w = fx_digit(t.val[e][sub >> 2], sub & 3);
+      if (w) __hip_atomic_fetch_add(acc + (size_t)key * 16 + sub, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  if (head && key >= 0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) atomic_add_fx(acc + (size_t)key * 4 + i, c[i]);
-  }
+  __syncthreads();
+  dep_table_clear(t);
+  __syncthreads();
+}
+
+// Exact 128-bit value of one accumulator slot (4 digit words).
+__device__ __forceinline__ fx128 dep_slot_value(const unsigned long long* w) {
+  // d0 + d1*2^32 + d2*2^64 + d3*2^96 in 128-bit two's complement arithmetic
+  const unsigned long long lo0 = w[0], mid = w[1], hi2 = w[2], top = w[3];
+  fx128 r{lo0, 0};
+  r = fx_add(r, fx128{mid << 32, (long long)(mid >> 32)});
+  r = fx_add(r, fx128{0ull, (long long)hi2});
+  r = fx_add(r, fx128{0ull, (long long)(top << 32)});
+  return r;
 }
 
 }  // namespace mpic

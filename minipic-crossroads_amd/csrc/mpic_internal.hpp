@@ -15,6 +15,7 @@ namespace mpic {
 constexpr int BLOCK = 256;         // threads per workgroup for particle and row kernels
 constexpr int WALK_CAP = 1000;     // crossings per particle per step (the reference has no cap)
 constexpr int ELL_SLICE = 64;      // rows per ELL slice = one wavefront
+constexpr int MAX_PARTIALS = 4096; // workgroups of the grid-stride particle kernels
 
 // 128-bit two's-complement fixed-point value (lo word first).
 struct fx128 {
@@ -78,10 +79,8 @@ struct PushPartial {
 };
 
 struct CGState {
-  int done, iters, converged, pad;
-  double rho0, rho, rho_old, alpha, beta, tol2;
-  unsigned int cnt_init, cnt_k1, cnt_k2, pad2;
-  int max_iter, pad3;
+  int done, iters, converged, max_iter;
+  double rho0, rho, tol2;
 };
 
 // ---------------------------------------------------------------------------
@@ -137,7 +136,7 @@ struct PushArgs {
   double dt, scale;
   int kill;
   int S_charge, S_ke;
-  fx128* acc_tet;          // [ne*4] charge accumulators (fused deposit), may be null
+  unsigned long long* acc; // [ne*16] element charge accumulators (4 nodes x 4 digit words)
   PushPartial* partial;    // [gridDim.x]
   double* E;               // transient [3][cap] for the split entry points
   double* vr;              // transient [3][cap]
@@ -154,20 +153,25 @@ hipError_t launch_ke(const DevParticles& P, int64_t n, const double* species, in
                      PushPartial* partial, hipStream_t s);
 
 hipError_t launch_deposit(const DevMesh& m, const DevParticles& P, int64_t n, const double* species, int S,
-                          fx128* acc_tet, hipStream_t s);
-hipError_t launch_node_charge(const DevMesh& m, fx128* acc_tet, int S, double* rhs, hipStream_t s);
-hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* species, unsigned long long* out2,
+                          unsigned long long* acc, hipStream_t s);
+hipError_t launch_node_charge(const DevMesh& m, unsigned long long* acc, int S, double* rhs, hipStream_t s);
+// max |w q| (ke_max_bits slot) and live count (crossings slot) into partials.
+hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* species, PushPartial* partial,
                                hipStream_t s);
+// Number of workgroup partials the particle kernels write for n particles.
+int partials_for(int64_t n);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 
 struct CGBuffers {
   double *x, *b, *r, *p0, *p1, *q;
   double *part_bb, *part_rr, *part_pq;
+  double* rho_hist;  // [max_iter+1]
   CGState* st;
   int nchunks;
 };
 hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_iter, hipStream_t s);
-hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, hipStream_t s);
+// Iteration it: convergence test + p/q update, then (it < max_iter) the x/r update.
+hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s);
 
 struct SortBuffers {
   int32_t* count;    // [ne+1]
