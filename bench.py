@@ -32,6 +32,18 @@ DENSITY = 2.0 / 3.0 * math.pi * math.pi  # main.cpp:68
 PUSH_BYTES_FP64 = 3 * 24 + 4 + 8 + 1 + 3 * 24 + 4
 
 
+def measured_traffic():
+    """HBM bytes per push launch measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs recorded in profiles/ (None when no such summary exists)."""
+    path = os.path.join(ROOT, "profiles", "k_push_fused_fp64_c2_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -116,15 +128,25 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--nppe", type=int, default=64)
     ap.add_argument("--dt", type=float, default=0.05)
-    ap.add_argument("--sort-period", type=int, default=4)
+    ap.add_argument("--sort-period", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 2 GiB-in/2 GiB-out elementwise kernel (torch) whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
               file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args)
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            torch.add(a, 1.0, out=b)  # an elementwise kernel: reads 2 GiB, writes 2 GiB
+        torch.cuda.synchronize()
+        log("copy calibration: 2 elementwise kernels of 2 GiB in, 2 GiB out done")
     npart, K = g["npart"], args.steps
     value = npart * K / g["wall"]
     push_ms = sum(g["push_ms"]) / len(g["push_ms"])
@@ -157,7 +179,7 @@ def main():
             "push_ms": push_ms,
         },
         "roofline": {"bound": "hbm", "kernel": "k_push_fused", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_particle": PUSH_BYTES_FP64},
         "cpu_baseline": cpu,
     }
