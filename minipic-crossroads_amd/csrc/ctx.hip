@@ -38,8 +38,11 @@ struct mpic_ctx {
   fx128* bgfix = nullptr;
   double* species = nullptr;  // [48]
 
-  // particles (two SoA sets for the sort)
-  int64_t cap = 0, n = 0;
+  // particles (two SoA sets: the fused push writes element-ordered output into the other one)
+  int64_t cap = 0, n = 0;   // n = positions in the current set P[cur]
+  int32_t* perm = nullptr;  // [cap] element order of P[cur] from the last ranking
+  bool perm_pending = false;
+  int64_t perm_n = 0;       // live particles listed in perm
   DevParticles P[2]{};
   int cur = 0;
   void* pmem[2] = {nullptr, nullptr};
@@ -63,6 +66,7 @@ struct mpic_ctx {
 
   // state
   bool acc_deposited = false;
+  bool sort_timed = false;  // T_SORT events hold a ranking not yet read
   int S = 100;
   double ke_max_prev = 0;
   int64_t steps = 0;
@@ -104,8 +108,8 @@ int dupload(mpic_ctx* c, T** p, const T* host, size_t count) {
   return MPIC_OK;
 }
 
-void tstart(mpic_ctx* c, int t) { (void)hipEventRecord(c->ev[2 * t], c->stream); }
-void tstop(mpic_ctx* c, int t) { (void)hipEventRecord(c->ev[2 * t + 1], c->stream); }
+void tstart(mpic_ctx* c, int t, hipStream_t s = nullptr) { (void)hipEventRecord(c->ev[2 * t], s ? s : c->stream); }
+void tstop(mpic_ctx* c, int t, hipStream_t s = nullptr) { (void)hipEventRecord(c->ev[2 * t + 1], s ? s : c->stream); }
 double telapsed(mpic_ctx* c, int t) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, c->ev[2 * t], c->ev[2 * t + 1]) != hipSuccess) return 0;
@@ -192,30 +196,59 @@ int free_particles(mpic_ctx* c) {
     }
   if (c->E) (void)hipFree(c->E);
   if (c->vr) (void)hipFree(c->vr);
+  if (c->perm) (void)hipFree(c->perm);
   c->E = c->vr = nullptr;
+  c->perm = nullptr;
+  c->perm_pending = false;
   c->cap = c->n = 0;
   return MPIC_OK;
 }
 
-int do_sort(mpic_ctx* c, int32_t* start_opt) {
-  const int src = c->cur, dst = 1 - c->cur;
+// Applies a pending ranking: the particle list becomes element-ordered and
+// compacted (dead particles dropped).
+int materialize(mpic_ctx* c) {
+  if (!c->perm_pending) return MPIC_OK;
+  HIPCHK(c, launch_permute(c->P[c->cur], c->P[1 - c->cur], c->perm, c->perm_n, c->stream));
+  c->cur = 1 - c->cur;
+  c->n = c->perm_n;
+  c->perm_pending = false;
+  return MPIC_OK;
+}
+
+// Counting sort of the current list as a permutation (applied by the next
+// fused push or by materialize).  n_live < 0: not known, read back start[ne].
+int rank_now(mpic_ctx* c, bool counts_ready, int64_t n_live) {
   tstart(c, T_SORT);
-  HIPCHK(c, launch_sort(c->m, c->P[src], c->P[dst], c->n, c->sb, c->stream));
+  HIPCHK(c, launch_rank(c->m, c->P[c->cur], c->n, counts_ready, c->sb, c->perm, c->stream));
   tstop(c, T_SORT);
-  HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + c->m.ne, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  if (n_live < 0) {
+    HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + c->m.ne, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    n_live = *c->start_h;
+  }
+  c->perm_n = n_live;
+  c->perm_pending = true;
+  c->sort_timed = true;
+  return MPIC_OK;
+}
+
+int do_sort(mpic_ctx* c, int32_t* start_opt) {
+  int rc;
+  if (!c->perm_pending && (rc = rank_now(c, false, -1))) return rc;
+  if ((rc = materialize(c))) return rc;
   if (start_opt)
     HIPCHK(c, hipMemcpyAsync(start_opt, c->sb.start, ((size_t)c->m.ne + 1) * sizeof(int32_t), hipMemcpyDeviceToHost,
                              c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->cur = dst;
-  c->n = *c->start_h;
   c->ms[T_SORT] = telapsed(c, T_SORT);
+  c->sort_timed = false;
   return MPIC_OK;
 }
 
 int do_weight_charge(mpic_ctx* c) {
-  if (!c->acc_deposited)
+  if (!c->acc_deposited) {
     HIPCHK(c, launch_deposit(c->m, c->P[c->cur], c->n, c->species, c->S, c->acc, c->stream));
+  }
   HIPCHK(c, launch_node_charge(c->m, c->acc, c->S, c->rhs, c->stream));
   c->acc_deposited = false;
   return MPIC_OK;
@@ -481,6 +514,7 @@ int mpic_set_form(mpic_ctx* c, int form) {
 
 int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
   if (!c || cap <= 0 || cap > ((int64_t)1 << 31) - 1) return fail(c, MPIC_ERR_ARG, "mpic_particles_reserve: bad capacity");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   free_particles(c);
   const size_t per = 10 * sizeof(double) + sizeof(int32_t) + sizeof(uint32_t) + 1;
   for (int k = 0; k < 2; ++k) {
@@ -497,9 +531,11 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
     P.id = (uint32_t*)(P.elem + cap);
     P.type = (uint8_t*)(P.id + cap);
   }
+  HIPCHK(c, hipMalloc((void**)&c->perm, sizeof(int32_t) * (size_t)cap));
   c->cap = cap;
   c->n = 0;
   c->cur = 0;
+  c->perm_pending = false;
   c->acc_deposited = false;
   c->ke_max_prev = 0;
   HIPCHK(c, hipMemset(c->acc, 0, (size_t)c->m.ne * 16 * sizeof(unsigned long long)));
@@ -507,6 +543,7 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
 }
 
 int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
+  if (c) c->perm_pending = false;  // the list is replaced
   if (!c || !h || n < 0 || !h->x || !h->x_ref || !h->v || !h->elem || !h->weight || !h->type || h->stride < n)
     return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: bad arguments");
   if (n > c->cap) return fail(c, MPIC_ERR_CAPACITY, "mpic_particles_upload: capacity exceeded");
@@ -537,7 +574,9 @@ int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
 }
 
 int mpic_particles_download(mpic_ctx* c, mpic_soa_host* h, int64_t* n) {
-  if (!c || !h || h->stride < c->n) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: bad arguments");
+  if (!c || !h) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: bad arguments");
+  if (materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
+  if (h->stride < c->n) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: stride below the particle count");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevParticles& P = c->P[c->cur];
   const int64_t cnt = c->n;
@@ -560,12 +599,13 @@ int mpic_particles_download(mpic_ctx* c, mpic_soa_host* h, int64_t* n) {
 
 int mpic_num_particles(mpic_ctx* c, int64_t* n) {
   if (!c || !n) return MPIC_ERR_ARG;
-  *n = c->n;
+  *n = c->perm_pending ? c->perm_n : c->n;
   return MPIC_OK;
 }
 
 int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel, double temp, uint64_t seed_offset,
               double pert_amp) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || nppe <= 0 || type < 0 || type >= c->cfg.num_species) return fail(c, MPIC_ERR_ARG, "mpic_fill: bad arguments");
   const int64_t add = (int64_t)c->m.ne * nppe;
   if (c->n + add > c->cap) return fail(c, MPIC_ERR_CAPACITY, "Cannot create that number of particles, no fill performed");
@@ -591,6 +631,7 @@ int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel
 }
 
 int mpic_weight_charge(mpic_ctx* c) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c) return MPIC_ERR_ARG;
   tstart(c, T_DEPOSIT);
   int rc = do_weight_charge(c);
@@ -612,6 +653,7 @@ int mpic_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres
 }
 
 int mpic_weight_efield(mpic_ctx* c) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_weight_efield: no particle list");
   int rc = ensure_transients(c);
   if (rc) return rc;
@@ -620,6 +662,7 @@ int mpic_weight_efield(mpic_ctx* c) {
 }
 
 int mpic_accel(mpic_ctx* c, double dt, double scale) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_accel: no particle list");
   int rc = ensure_transients(c);
   if (rc) return rc;
@@ -631,6 +674,7 @@ int mpic_accel(mpic_ctx* c, double dt, double scale) {
 }
 
 int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_move: no particle list");
   int rc = ensure_transients(c);
   if (rc) return rc;
@@ -650,6 +694,7 @@ int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
 }
 
 int mpic_compute_ke(mpic_ctx* c, double* ke) {
+  if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || !ke) return MPIC_ERR_ARG;
   if (c->n == 0) {
     *ke = 0;
@@ -679,7 +724,8 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_time_step: no particle list");
   int rc;
   mpic_step_stats st{};
-  // 1. charge of the current positions (PIC.cpp:250)
+  // 1. charge of the current positions (PIC.cpp:250); a pending ranking does not
+  //    matter here (the sums are order-free)
   tstart(c, T_DEPOSIT);
   if ((rc = do_weight_charge(c))) return rc;
   tstop(c, T_DEPOSIT);
@@ -688,31 +734,50 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   rc = do_solve(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres);
   tstop(c, T_SOLVE);
   if (rc) return rc;
-  // 3. fused gather + kick + walk + deposit of the new positions + KE (PIC.cpp:252-258)
+  // 3. fused gather + kick + walk + deposit of the new positions + KE (PIC.cpp:252-258);
+  //    with a pending ranking the push reads through it and writes element order
   if (!(c->ke_max_prev > 0) && c->n > 0) {
     PushPartial r{};
     HIPCHK(c, launch_ke(c->P[c->cur], c->n, c->species, 0, 0, c->partial, c->stream));
     if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
     std::memcpy(&c->ke_max_prev, &r.ke_max_bits, 8);
   }
+  const bool rank_after = c->cfg.sort_period > 0 && (c->steps + 1) % c->cfg.sort_period == 0;
   PushArgs a = push_args(c);
   a.dt = dt;
-  a.S_ke = scale_for_bound((double)c->n * c->ke_max_prev * 256.0);
+  a.S_ke = scale_for_bound((double)(c->perm_pending ? c->perm_n : c->n) * c->ke_max_prev * 256.0);
+  if (c->perm_pending) {
+    a.perm = c->perm;
+    a.Q = c->P[1 - c->cur];
+    a.n = c->perm_n;
+  }
+  if (rank_after) {
+    HIPCHK(c, launch_zero(c->sb.count, (size_t)(((int64_t)c->m.ne + 2) / 2) * 8, c->stream));
+    a.count = c->sb.count;
+  }
   tstart(c, T_PUSH);
   tstart(c, T_PUSH_KERNEL);
   HIPCHK(c, launch_push_fused(a, c->stream));
   tstop(c, T_PUSH_KERNEL);
+  if (c->perm_pending) {
+    c->cur = 1 - c->cur;
+    c->n = c->perm_n;
+    c->perm_pending = false;
+  }
   PushPartial r{};
   if (c->n > 0) {
     if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
   }
   tstop(c, T_PUSH);
   c->acc_deposited = true;
+  if (c->sort_timed) {  // the previous step's ranking has completed
+    c->ms[T_SORT] = telapsed(c, T_SORT);
+    c->sort_timed = false;
+  }
   double mx;
   std::memcpy(&mx, &r.ke_max_bits, 8);
-  const int64_t n_before = c->n;
   tstart(c, T_KE);
-  const int S_safe = scale_for_bound((double)n_before * mx);
+  const int S_safe = scale_for_bound((double)c->n * mx);
   if (a.S_ke > S_safe) {
     // The provisional scale could have overflowed: sum again at a safe scale.
     PushPartial k{};
@@ -726,17 +791,25 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   c->ke_max_prev = mx;
   st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
   c->steps++;
-  // 4. remove killed particles; periodic counting sort by element
-  c->ms[T_SORT] = 0;
-  if (r.killed > 0 || (c->cfg.sort_period > 0 && c->steps % c->cfg.sort_period == 0))
-    if ((rc = do_sort(c, nullptr))) return rc;
+  // 4. ranking by element of the new positions (counting sort; the counts came
+  //    from the push), also whenever particles were killed (cleanup_list)
+  const int64_t n_live = c->n - r.killed;
+  if (rank_after) {
+    if ((rc = rank_now(c, true, n_live))) return rc;
+  } else if (r.killed > 0) {
+    if ((rc = rank_now(c, false, n_live))) return rc;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
   c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
   c->ms[T_PUSH] = telapsed(c, T_PUSH);
   c->ms[T_PUSH_KERNEL] = telapsed(c, T_PUSH_KERNEL);
   c->ms[T_KE] = telapsed(c, T_KE);
-  st.num_particles = c->n;
+  if (c->sort_timed) {
+    c->ms[T_SORT] = telapsed(c, T_SORT);
+    c->sort_timed = false;
+  }
+  st.num_particles = c->perm_pending ? c->perm_n : c->n;
   st.charge_scale_bits = c->S;
   st.ms_deposit = c->ms[T_DEPOSIT];
   st.ms_solve = c->ms[T_SOLVE];

@@ -1,6 +1,9 @@
 // Particle kernels of the MiniPIC timestep on gfx950:
 //   fused push  = E gather + Boris kick + barycentric tet walk + charge deposit of
-//                 the new position + kinetic energy, one thread per particle;
+//                 the new position + kinetic energy + element counts of the new
+//                 positions, one thread per particle; when a permutation from the
+//                 previous step's ranking is given, each thread gathers its
+//                 particle through it and writes the result in element order;
 //   split phases = the same arithmetic as separate launches (weight_Efield,
 //                 accel, move) for the reference's phase-by-phase API.
 // All FP64 expressions keep the reference's association order and are compiled
@@ -217,7 +220,10 @@ __device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsign
 
 struct KArgs {
   DevMesh m;
-  DevParticles P;
+  DevParticles P;   // input particles
+  DevParticles Q;   // output particles of the fused push (== P when pushing in place)
+  const int32_t* perm;  // fused push: output position -> input position, or null (in place)
+  int32_t* count;       // fused push: per-element counts of the new positions, or null
   int64_t n;
   const double* phi;
   double dt, scale;
@@ -246,6 +252,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   __shared__ DepTable table;
   dep_table_clear(table);
   const int64_t cap = a.P.cap;
+  const int64_t qcap = a.Q.cap;
+  const bool gather = a.perm != nullptr;
   WalkCount wc{0, 0, 0, 0};
   long long killed = 0;
   fx128 ke{0, 0};
@@ -255,22 +263,23 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   __syncthreads();
   int chunk = 0;
   for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
-    const int64_t p = base + threadIdx.x;
+    const int64_t p = base + threadIdx.x;  // output position
     int enew = -1;
     double w = 0, V[3] = {0, 0, 0};
     int t = 0;
     if (p < hi) {
-      int e = a.P.elem[p];
+      const int64_t src = gather ? (int64_t)a.perm[p] : p;  // sorted order: gather through the permutation
+      int e = a.P.elem[src];
       if (e >= 0) {
         double X[3], R[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          X[d] = a.P.x[d * cap + p];
-          R[d] = a.P.xr[d * cap + p];
-          V[d] = a.P.v[d * cap + p];
+          X[d] = a.P.x[d * cap + src];
+          R[d] = a.P.xr[d * cap + src];
+          V[d] = a.P.v[d * cap + src];
         }
-        t = a.P.type[p];
-        w = a.P.w[p];
+        t = a.P.type[src];
+        w = a.P.w[src];
         double E[3];
         gather_E(a.m, a.phi, e, R, E);
         double g[12];
@@ -280,27 +289,34 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          a.P.x[d * cap + p] = X[d];
-          a.P.xr[d * cap + p] = R[d];
-          a.P.v[d * cap + p] = V[d];
+          a.Q.x[d * qcap + p] = X[d];
+          a.Q.xr[d * qcap + p] = R[d];
+          a.Q.v[d * qcap + p] = V[d];
         }
         enew = alive ? e : -1;
         killed += alive ? 0 : 1;
-        a.P.elem[p] = enew;
+        a.Q.elem[p] = enew;
+        if (gather) {
+          a.Q.w[p] = w;
+          a.Q.type[p] = (uint8_t)t;
+          a.Q.id[p] = a.P.id[src];
+        }
+      } else if (gather) {
+        a.Q.elem[p] = -1;
       }
     }
     if (enew >= 0) {
       fx128 c[4];
       particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
-      dep_table_add(table, a.acc, enew, c);
+      dep_table_add(table, a.acc, a.count, enew, c);
       const double term = ke_term(w, a.sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
       const unsigned long long tb = __double_as_longlong(term);
       kmax = tb > kmax ? tb : kmax;
     }
-    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, a.acc);
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, a.acc, a.count);
   }
-  dep_table_flush(table, a.acc);
+  dep_table_flush(table, a.acc, a.count);
   block_partial(a.partial, ke, kmax, wc, killed);
 }
 
@@ -383,12 +399,12 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, in
       if (e >= 0) {
         fx128 c[4];
         particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
-        dep_table_add(table, acc, e, c);
+        dep_table_add(table, acc, nullptr, e, c);
       }
     }
-    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, acc);
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, acc, nullptr);
   }
-  dep_table_flush(table, acc);
+  dep_table_flush(table, acc, nullptr);
 }
 
 // KE terms (PIC.cpp:191-200): the per-workgroup max term, the live count (in
@@ -462,8 +478,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, 
 }
 
 KArgs kargs(const PushArgs& a) {
-  return KArgs{a.m, a.P, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge, a.S_ke, a.acc, a.partial, a.E, a.vr,
-               a.species};
+  return KArgs{a.m, a.P, a.perm ? a.Q : a.P, a.perm, a.count, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge,
+               a.S_ke, a.acc, a.partial, a.E, a.vr, a.species};
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }

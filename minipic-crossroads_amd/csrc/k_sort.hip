@@ -1,14 +1,17 @@
-// Counting sort of the particle SoA by owning element: the build's replacement
+// Counting sort of the particle list by owning element: the build's replacement
 // for ParticleList::sort (Kokkos::BinSort, particle_list.cpp:90-116) and
 // compute_start_particles_by_element (particle_list.cpp:118-146).
-//   1. count[e] = particles in element e (wave-aggregated atomics);
+//   1. count[e] = particles in element e (key-grouped atomics; the fused push
+//      produces the counts itself);
 //   2. start = exclusive scan of count (start[ne] = live particles);
-//   3. scatter every live particle to start[e] + rank; dead particles
-//      (elem < 0) are dropped, which also compacts the list
+//   3. rank: perm[start[e] + rank] = position of every live particle; dead
+//      particles (elem < 0) get no slot, which also compacts the list
 //      (DeleteParticles, particle_list.cpp:148-191).
-// As in the reference, the order inside one element is not specified (it
-// depends on the order of the per-wavefront atomics); the identity key (id)
-// travels with the particle.
+// Only the 4-byte permutation is scattered: the next fused push gathers each
+// particle through it and writes the whole state in element order (or
+// launch_permute applies it).  As in the reference, the order inside one element
+// is not specified (it follows the order of the per-wavefront atomics); the
+// identity key (id) travels with the particle.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
 
@@ -104,11 +107,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scatter(DevParticles src, DevParticles dst, int64_t n,
-                                                   int32_t* __restrict__ cursor) {
+__global__ __launch_bounds__(BLOCK) void k_rank(const int32_t* __restrict__ elem, int64_t n,
+                                                int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int e = p < n ? src.elem[p] : -1;
+  const int e = p < n ? elem[p] : -1;
   unsigned long long grp;
   const int leader = wave_group_leader(e, &grp);
   int base = 0;
@@ -116,7 +119,14 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(DevParticles src, DevParticle
   base = __shfl(base, leader < 0 ? lane : leader);
   if (e < 0) return;
   const unsigned long long below = (lane == 0) ? 0ull : (grp & ((1ull << lane) - 1ull));
-  const int64_t q = base + __popcll(below);
+  perm[base + __popcll(below)] = (int32_t)p;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_permute(DevParticles src, DevParticles dst, const int32_t* __restrict__ perm,
+                                                   int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const int64_t p = perm[q];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     dst.x[d * dst.cap + q] = src.x[d * src.cap + p];
@@ -124,26 +134,35 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(DevParticles src, DevParticle
     dst.v[d * dst.cap + q] = src.v[d * src.cap + p];
   }
   dst.w[q] = src.w[p];
-  dst.elem[q] = e;
+  dst.elem[q] = src.elem[p];
   dst.type[q] = src.type[p];
   dst.id[q] = src.id[p];
 }
 
 }  // namespace
 
-hipError_t launch_sort(const DevMesh& m, const DevParticles& src, const DevParticles& dst, int64_t n,
-                       SortBuffers& sb, hipStream_t s) {
+hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool counts_ready, SortBuffers& sb,
+                       int32_t* perm, hipStream_t s) {
   const int64_t ne1 = (int64_t)m.ne + 1;
-  hipError_t err = launch_zero(sb.count, (size_t)((ne1 + 1) / 2) * 8, s);
-  if (err != hipSuccess) return err;
   const unsigned gp = (unsigned)((n + BLOCK - 1) / BLOCK);
-  if (n > 0) hipLaunchKernelGGL(k_count, dim3(gp), dim3(BLOCK), 0, s, src.elem, n, sb.count);
+  if (!counts_ready) {
+    hipError_t err = launch_zero(sb.count, (size_t)((ne1 + 1) / 2) * 8, s);
+    if (err != hipSuccess) return err;
+    if (n > 0) hipLaunchKernelGGL(k_count, dim3(gp), dim3(BLOCK), 0, s, P.elem, n, sb.count);
+  }
   const int ntiles = (int)((ne1 + SCAN_TILE - 1) / SCAN_TILE);
   hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, sb.blocksum, ntiles);
   hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum, sb.start, sb.cursor,
                      (int64_t)m.ne);
-  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(gp), dim3(BLOCK), 0, s, src, dst, n, sb.cursor);
+  if (n > 0) hipLaunchKernelGGL(k_rank, dim3(gp), dim3(BLOCK), 0, s, P.elem, n, sb.cursor, perm);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(const DevParticles& src, const DevParticles& dst, const int32_t* perm, int64_t n,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, src, dst, perm, n);
   return hipGetLastError();
 }
 

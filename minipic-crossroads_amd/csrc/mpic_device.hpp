@@ -100,12 +100,14 @@ constexpr int DEP_EMPTY = -1;
 
 struct DepTable {
   int key[DEP_TABLE];
+  int cnt[DEP_TABLE];       // particles added to the entry (element counts for the sort)
   fx128 val[DEP_TABLE][4];
 };
 
 __device__ __forceinline__ void dep_table_clear(DepTable& t) {
   for (int k = threadIdx.x; k < DEP_TABLE; k += BLOCK) {
     t.key[k] = DEP_EMPTY;
+    t.cnt[k] = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) t.val[k][i] = fx128{0, 0};
   }
@@ -121,8 +123,10 @@ __device__ __forceinline__ unsigned long long fx_digit(fx128 v, int d) {
   }
 }
 
-// Direct global add of one particle's 4 node charges (table overflow path).
-__device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ acc, int key, const fx128 c[4]) {
+// Direct global add of one particle's 4 node charges and count (table overflow path).
+__device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ acc, int* __restrict__ count, int key,
+                                               const fx128 c[4]) {
+  if (count) atomicAdd(count + key, 1);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -133,9 +137,10 @@ __device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ 
 }
 
 // Adds one particle's node charges to the table entry of element key (key < 0:
-// nothing).  Uses LDS atomics; falls back to global atomics if the table is full.
-__device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* __restrict__ acc, int key,
-                                              const fx128 c[4]) {
+// nothing) and counts it.  Uses LDS atomics; falls back to global atomics if the
+// table is full.
+__device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* __restrict__ acc,
+                                              int* __restrict__ count, int key, const fx128 c[4]) {
   if (key < 0) return;
   unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);  // 9 = log2(DEP_TABLE)
   for (int probe = 0; probe < DEP_TABLE; ++probe) {
@@ -149,16 +154,18 @@ __device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* _
     if (k == key) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) lds_add_fx(&t.val[h][i], c[i]);
+      atomicAdd(&t.cnt[h], 1);
       return;
     }
     h = (h + 1) & (DEP_TABLE - 1);
   }
-  dep_global_add(acc, key, c);
+  dep_global_add(acc, count, key, c);
 }
 
-// Flushes every used entry with coalesced non-returning atomics and clears the
-// table.  Must be called by the whole workgroup.
-__device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long* __restrict__ acc) {
+// Flushes every used entry with coalesced non-returning atomics (charges; counts
+// when count != null) and clears the table.  Must be called by the whole workgroup.
+__device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long* __restrict__ acc,
+                                                int* __restrict__ count) {
   __syncthreads();
   const int sub = threadIdx.x & 15;  // node = sub/4, digit = sub%4
   for (int e = threadIdx.x >> 4; e < DEP_TABLE; e += BLOCK / 16) {
@@ -166,6 +173,7 @@ __device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long*
     if (key != DEP_EMPTY) {
       const unsigned long long w = fx_digit(t.val[e][sub >> 2], sub & 3);
       if (w) __hip_atomic_fetch_add(acc + (size_t)key * 16 + sub, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (count && sub == 0) atomicAdd(count + key, t.cnt[e]);
     }
   }
   __syncthreads();

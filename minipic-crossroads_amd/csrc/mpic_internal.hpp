@@ -130,7 +130,10 @@ inline int scale_for_bound(double bound) {
 // ---------------------------------------------------------------------------
 struct PushArgs {
   DevMesh m;
-  DevParticles P;
+  DevParticles P;          // input particles
+  DevParticles Q;          // output (fused push with perm)
+  const int32_t* perm;     // output position -> input position, or null (push in place)
+  int32_t* count;          // element counts of the new positions (fused push), or null
   int64_t n;
   const double* phi;
   double dt, scale;
@@ -180,8 +183,13 @@ struct SortBuffers {
   int32_t* blocksum; // scan scratch
   int scan_blocks;
 };
-hipError_t launch_sort(const DevMesh& m, const DevParticles& src, const DevParticles& dst, int64_t n,
-                       SortBuffers& sb, hipStream_t s);
+// Counting sort by element, as a permutation: count (unless counts_ready), scan
+// into sb.start, then perm[start[e] + rank] = position of each live particle.
+hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool counts_ready, SortBuffers& sb,
+                       int32_t* perm, hipStream_t s);
+// dst[i] = src[perm[i]] for every field, i < n.
+hipError_t launch_permute(const DevParticles& src, const DevParticles& dst, const int32_t* perm, int64_t n,
+                          hipStream_t s);
 
 struct FillArgs {
   DevMesh m;
