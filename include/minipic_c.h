@@ -69,7 +69,8 @@ typedef struct {
   double cg_tol;                         /* relative residual tolerance (ES::solve(tol), ES.cpp:216) */
   int cg_max_iter;                       /* CG_solve.hpp:73 uses 1000 */
   int kill_on_wall;                      /* DataWarehouse::set_kill_particles_on_wall_impact, data_warehouse.hpp:76 */
-  int sort_period;                       /* counting sort by element every N time steps (0 = never) */
+  int sort_period;                       /* reorder the particle list every N time steps (0 = never) */
+  int sort_bucket;                       /* elements per spatial bucket of that reordering (<= 1: by element) */
   int num_species;                       /* ParticleTypeList, particle_type.hpp:58-86 */
   double species_q[MPIC_MAX_SPECIES];
   double species_m[MPIC_MAX_SPECIES];
@@ -93,6 +94,7 @@ typedef struct {
   int64_t capped;      /* particles that hit the crossing cap (reference: unbounded loop) */
   int64_t nonpositive; /* sub-steps with s <= 0 (the reference asserts s > 0, particle_move.hpp:223) */
   int64_t reflections; /* specular wall reflections (particle_move.hpp:213-220) */
+  int64_t relocated;   /* precision form: element changes made by the FP64 point-location test */
 } mpic_move_stats;
 
 typedef struct {
@@ -106,8 +108,9 @@ typedef struct {
 } mpic_step_stats;
 
 /* ---- configuration -------------------------------------------------------- */
-/* Defaults: FP64 form, CG tol 1e-10, max_iter 1000, reflect on walls, sort every
- * 4 steps, species e-(-1,1) e+(1,1) H+(1,1837) as particle_type.hpp:65-74. */
+/* Defaults: FP64 form, CG tol 1e-10, max_iter 1000, reflect on walls, reorder
+ * by element every step, species e-(-1,1) e+(1,1) H+(1,1837) as
+ * particle_type.hpp:65-74. */
 void mpic_config_default(mpic_config* cfg);
 const char* mpic_last_error(const mpic_ctx* ctx); /* ctx may be NULL: last create/mesh error */
 
@@ -127,6 +130,13 @@ void mpic_mesh_free(mpic_mesh* mesh);
 /* DataWarehouse::create_mesh/create_ES + PIC::PIC upload (data_warehouse.cpp:50-73). */
 int mpic_create(const mpic_mesh_desc* mesh, const mpic_config* cfg, mpic_ctx** out);
 void mpic_destroy(mpic_ctx* ctx);
+/* MPIC_FORM_FP64: the drop-in (bit-identical to the reference's arithmetic).
+ * MPIC_FORM_MIXED: FP32 particle state, push, walk and gather; FP64 point location
+ * near faces; 64-bit fixed-point element deposit; FP32 CG inside FP64 iterative
+ * refinement to the same |r| < tol |b| rule.  Switching converts the particle
+ * state.  In the mixed form the phase-by-phase calls weight_efield, accel and
+ * move return MPIC_ERR_STATE; weight_charge, solve, compute_ke, sort and
+ * time_step run in that form. */
 int mpic_set_form(mpic_ctx* ctx, int form);
 
 /* ---- particles (ParticleList, particle_list.hpp:54-320) -------------------- */
@@ -156,6 +166,10 @@ int mpic_get_phi(mpic_ctx* ctx, double* phi);        /* ES::phi(lid), ES.hpp:58 
 int mpic_set_phi(mpic_ctx* ctx, const double* phi);
 int mpic_get_efield(mpic_ctx* ctx, double* E, int64_t stride); /* ParticleList::E after weight_efield, [3][stride] */
 int mpic_charge_scale(mpic_ctx* ctx, int* bits);
+/* Live particles whose element does not contain their position under the FP64
+ * barycentric test (coordinate below -1e-9 across an interior face): the
+ * ownership-mismatch count of the precision form (0 by construction). */
+int mpic_check_ownership(mpic_ctx* ctx, int64_t* mismatches);
 
 /* ---- instrumentation (PIC::output_timers, PIC.hpp:85-88) --------------------- */
 /* Average device duration (ms) of the last launch of the named kernel family:

@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "mpic_internal.hpp"
+#include "mpic_ctx.hpp"
 
 namespace mpic {
 std::string& global_error() {
@@ -19,101 +19,47 @@ std::string& global_error() {
 
 using namespace mpic;
 
-namespace {
+namespace mpic {
 
-enum Timer { T_DEPOSIT, T_SOLVE, T_PUSH, T_SORT, T_KE, T_PUSH_KERNEL, T_COUNT };
-
-}  // namespace
-
-struct mpic_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  mpic_config cfg{};
-  std::string err;
-  std::vector<void*> allocs;  // mesh-lifetime device buffers
-
-  DevMesh m{};
-  std::vector<double> background;  // host copy
-  double bg_abs_sum = 0;
-  fx128* bgfix = nullptr;
-  double* species = nullptr;  // [48]
-
-  // particles (two SoA sets: the fused push writes element-ordered output into the other one)
-  int64_t cap = 0, n = 0;   // n = positions in the current set P[cur]
-  int32_t* perm = nullptr;  // [cap] element order of P[cur] from the last ranking
-  bool perm_pending = false;
-  int64_t perm_n = 0;       // live particles listed in perm
-  DevParticles P[2]{};
-  int cur = 0;
-  void* pmem[2] = {nullptr, nullptr};
-  double* E = nullptr;   // transient [3][cap]
-  double* vr = nullptr;  // transient [3][cap]
-  PushPartial* partial = nullptr;
-  PushPartial* finish = nullptr;   // device [1]
-  PushPartial* finish_h = nullptr; // pinned host [1]
-
-  // fields
-  double* phi = nullptr;
-  double* rhs = nullptr;
-  unsigned long long* acc = nullptr;  // [ne*16] element charge digits
-  CGBuffers cg{};
-  CGState* cg_h = nullptr;  // pinned
-  int last_cg_iters = 0;
-  int rho_hist_cap = 0;
-
-  SortBuffers sb{};
-  int32_t* start_h = nullptr;  // pinned [1]
-
-  // state
-  bool acc_deposited = false;
-  bool sort_timed = false;  // T_SORT events hold a ranking not yet read
-  int S = 100;
-  double ke_max_prev = 0;
-  int64_t steps = 0;
-  hipEvent_t ev[2 * T_COUNT] = {};
-  double ms[T_COUNT] = {};
-};
-
-namespace {
-
-int fail(mpic_ctx* c, int code, const std::string& msg) {
-  if (c)
-    c->err = msg;
-  else
-    global_error() = msg;
-  return code;
-}
-
-#define HIPCHK(c, call)                                                                           \
-  do {                                                                                            \
-    hipError_t e_ = (call);                                                                       \
-    if (e_ != hipSuccess) return fail(c, MPIC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class T>
-int dalloc(mpic_ctx* c, T** p, size_t count) {
-  void* q = nullptr;
-  if (count == 0) count = 1;
-  HIPCHK(c, hipMalloc(&q, count * sizeof(T)));
-  c->allocs.push_back(q);
-  *p = (T*)q;
-  return MPIC_OK;
-}
-
-template <class T>
-int dupload(mpic_ctx* c, T** p, const T* host, size_t count) {
-  int rc = dalloc(c, p, count);
-  if (rc) return rc;
-  if (count) HIPCHK(c, hipMemcpy(*p, host, count * sizeof(T), hipMemcpyHostToDevice));
-  return MPIC_OK;
-}
-
-void tstart(mpic_ctx* c, int t, hipStream_t s = nullptr) { (void)hipEventRecord(c->ev[2 * t], s ? s : c->stream); }
-void tstop(mpic_ctx* c, int t, hipStream_t s = nullptr) { (void)hipEventRecord(c->ev[2 * t + 1], s ? s : c->stream); }
-double telapsed(mpic_ctx* c, int t) {
-  float ms = 0;
-  if (hipEventElapsedTime(&ms, c->ev[2 * t], c->ev[2 * t + 1]) != hipSuccess) return 0;
-  return ms;
+// Spatial buckets for the timestep's particle ordering: the cells of a uniform
+// grid over the node bounding box, about `per` elements each, holding the
+// elements whose centroid lies in them, numbered in Morton order of the cell
+// (nearby buckets are nearby in memory).  per <= 1: one bucket per element.
+std::vector<int32_t> make_buckets(const mpic_mesh_desc* d, const std::vector<int32_t>& conn, int per,
+                                  const double lo[3], const double hi[3], int32_t* nb) {
+  const int32_t ne = d->num_elems;
+  std::vector<int32_t> b(ne);
+  if (per <= 1) {
+    for (int32_t e = 0; e < ne; ++e) b[e] = e;
+    *nb = ne;
+    return b;
+  }
+  double L[3], vol = 1.0;
+  for (int a = 0; a < 3; ++a) {
+    L[a] = std::max(hi[a] - lo[a], 1e-300);
+    vol *= L[a];
+  }
+  const double side = std::cbrt(vol * per / ne);
+  int cells[3];
+  for (int a = 0; a < 3; ++a) cells[a] = (int)std::min(1024.0, std::max(1.0, std::round(L[a] / side)));
+  std::vector<uint32_t> code(ne);
+  for (int32_t e = 0; e < ne; ++e) {
+    uint32_t m = 0;
+    for (int a = 0; a < 3; ++a) {
+      double cen = 0;
+      for (int k = 0; k < 4; ++k) cen += d->nodes[(size_t)conn[(size_t)e * 4 + k] * 3 + a];
+      int q = (int)std::floor((cen * 0.25 - lo[a]) / L[a] * cells[a]);
+      q = std::min(std::max(q, 0), cells[a] - 1);
+      for (int bit = 0; bit < 10; ++bit) m |= (uint32_t)((q >> bit) & 1) << (3 * bit + a);
+    }
+    code[e] = m;
+  }
+  std::vector<uint32_t> u(code);
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  for (int32_t e = 0; e < ne; ++e) b[e] = (int32_t)(std::lower_bound(u.begin(), u.end(), code[e]) - u.begin());
+  *nb = (int32_t)u.size();
+  return b;
 }
 
 PushArgs push_args(mpic_ctx* c) {
@@ -146,6 +92,7 @@ int finish_partials(mpic_ctx* c, int parts, PushPartial* out) {
 
 // Charge fixed-point scale from the particles: bound = sum|bg| + n_live * max|w q|.
 int update_charge_scale(mpic_ctx* c) {
+  if (c->state32) return mixed_charge_scale(c);
   PushPartial r{};
   if (c->n > 0) {
     HIPCHK(c, launch_charge_bound(c->P[c->cur], c->n, c->species, c->partial, c->stream));
@@ -177,6 +124,10 @@ int invalidate_deposit(mpic_ctx* c) {
     HIPCHK(c, launch_zero(c->acc, (size_t)c->m.ne * 16 * sizeof(unsigned long long), c->stream));
     c->acc_deposited = false;
   }
+  if (c->acc64_deposited) {
+    HIPCHK(c, launch_zero(c->acc64, (size_t)c->m.ne * sizeof(unsigned long long), c->stream));
+    c->acc64_deposited = false;
+  }
   return MPIC_OK;
 }
 
@@ -200,6 +151,7 @@ int free_particles(mpic_ctx* c) {
   c->E = c->vr = nullptr;
   c->perm = nullptr;
   c->perm_pending = false;
+  mixed_free_particles(c);
   c->cap = c->n = 0;
   return MPIC_OK;
 }
@@ -208,6 +160,13 @@ int free_particles(mpic_ctx* c) {
 // compacted (dead particles dropped).
 int materialize(mpic_ctx* c) {
   if (!c->perm_pending) return MPIC_OK;
+  if (c->state32) {
+    HIPCHK(c, launch_permute32(c->P32[c->cur32], c->P32[1 - c->cur32], c->perm, c->perm_n, c->stream));
+    c->cur32 = 1 - c->cur32;
+    c->n = c->perm_n;
+    c->perm_pending = false;
+    return MPIC_OK;
+  }
   HIPCHK(c, launch_permute(c->P[c->cur], c->P[1 - c->cur], c->perm, c->perm_n, c->stream));
   c->cur = 1 - c->cur;
   c->n = c->perm_n;
@@ -216,25 +175,32 @@ int materialize(mpic_ctx* c) {
 }
 
 // Counting sort of the current list as a permutation (applied by the next
-// fused push or by materialize).  n_live < 0: not known, read back start[ne].
-int rank_now(mpic_ctx* c, bool counts_ready, int64_t n_live) {
+// fused push or by materialize), by element or by sort bucket.  n_live < 0: not
+// known, read back from the scan.
+int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live) {
+  DevParticles live = c->P[c->cur];  // only the element array is read
+  if (c->state32) live.elem = c->P32[c->cur32].elem;
   tstart(c, T_SORT);
-  HIPCHK(c, launch_rank(c->m, c->P[c->cur], c->n, counts_ready, c->sb, c->perm, c->stream));
+  HIPCHK(c, launch_rank(c->m, live, c->n, by_bucket, counts_ready, c->sb, c->perm, c->stream));
   tstop(c, T_SORT);
   if (n_live < 0) {
-    HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + c->m.ne, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    const int32_t nkeys = by_bucket ? c->m.nbuckets : c->m.ne;
+    HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + nkeys, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     n_live = *c->start_h;
   }
   c->perm_n = n_live;
   c->perm_pending = true;
+  c->perm_by_bucket = by_bucket;
   c->sort_timed = true;
   return MPIC_OK;
 }
 
+// Element order (ParticleList::sort + compute_start_particles_by_element).
 int do_sort(mpic_ctx* c, int32_t* start_opt) {
   int rc;
-  if (!c->perm_pending && (rc = rank_now(c, false, -1))) return rc;
+  if (c->perm_pending && c->perm_by_bucket && (rc = materialize(c))) return rc;
+  if (!c->perm_pending && (rc = rank_now(c, false, false, -1))) return rc;
   if ((rc = materialize(c))) return rc;
   if (start_opt)
     HIPCHK(c, hipMemcpyAsync(start_opt, c->sb.start, ((size_t)c->m.ne + 1) * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -284,7 +250,10 @@ int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
   return MPIC_OK;
 }
 
-}  // namespace
+}  // namespace mpic
+
+#define FP64_ONLY \
+  "the precision form runs the fused time step; select MPIC_FORM_FP64 for the phase-by-phase API"
 
 extern "C" {
 
@@ -294,7 +263,8 @@ void mpic_config_default(mpic_config* cfg) {
   cfg->cg_tol = 1e-10;
   cfg->cg_max_iter = 1000;
   cfg->kill_on_wall = 0;
-  cfg->sort_period = 4;
+  cfg->sort_period = 1;
+  cfg->sort_bucket = 1;
   cfg->num_species = 3;
   for (int t = 0; t < MPIC_MAX_SPECIES; ++t) cfg->species_m[t] = 1.0;
   cfg->species_q[0] = -1.0;  // e-  (particle_type.hpp:69-71)
@@ -375,6 +345,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     ell_len[s] = L;
     ell_off[s + 1] = ell_off[s] + (int64_t)L * ELL_SLICE;
   }
+  c->ell_entries = ell_off[nsl];
   std::vector<int32_t> ell_col(ell_off[nsl]);
   std::vector<double> ell_val(ell_off[nsl]);
   for (int32_t s = 0; s < nsl; ++s)
@@ -452,6 +423,14 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       c->m.lo[a] = std::min(c->m.lo[a], d->nodes[(size_t)i * 3 + a]);
       c->m.hi[a] = std::max(c->m.hi[a], d->nodes[(size_t)i * 3 + a]);
     }
+  {
+    int32_t nb = 0;
+    std::vector<int32_t> bk = make_buckets(d, conn, c->cfg.sort_bucket, c->m.lo, c->m.hi, &nb);
+    int32_t* dbk;
+    if ((rc = dupload(c, &dbk, bk.data(), bk.size()))) return bail(rc);
+    c->m.bucket = dbk;
+    c->m.nbuckets = nb;
+  }
   if (d->background) {
     c->background.assign(d->background, d->background + nn);
     for (int32_t i = 0; i < nn; ++i) c->bg_abs_sum += std::fabs(c->background[i]);
@@ -485,6 +464,9 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       hipMemset(c->cg.st, 0, sizeof(CGState)) != hipSuccess)
     return bail(fail(c, MPIC_ERR_HIP, "hipMemset failed"));
   if ((rc = update_charge_scale(c))) return bail(rc);
+  if (c->cfg.form == MPIC_FORM_MIXED && (rc = mixed_prepare(c))) return bail(rc);
+  if (c->cfg.form != MPIC_FORM_FP64 && c->cfg.form != MPIC_FORM_MIXED)
+    return bail(fail(c, MPIC_ERR_ARG, "mpic_create: unknown form"));
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MPIC_ERR_HIP, "setup synchronisation failed"));
   *out = c;
   return MPIC_OK;
@@ -498,6 +480,9 @@ void mpic_destroy(mpic_ctx* c) {
   if (c->cg.rho_hist) (void)hipFree(c->cg.rho_hist);
   if (c->cg_h) (void)hipHostFree(c->cg_h);
   if (c->finish_h) (void)hipHostFree(c->finish_h);
+  if (c->ir_h) (void)hipHostFree(c->ir_h);
+  if (c->bad_h) (void)hipHostFree(c->bad_h);
+  if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
   if (c->start_h) (void)hipHostFree(c->start_h);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -507,7 +492,10 @@ void mpic_destroy(mpic_ctx* c) {
 
 int mpic_set_form(mpic_ctx* c, int form) {
   if (!c) return MPIC_ERR_ARG;
-  if (form != MPIC_FORM_FP64) return fail(c, MPIC_ERR_ARG, "mpic_set_form: form not available");
+  if (form != MPIC_FORM_FP64 && form != MPIC_FORM_MIXED) return fail(c, MPIC_ERR_ARG, "mpic_set_form: unknown form");
+  int rc;
+  if (form == MPIC_FORM_MIXED && (rc = mixed_prepare(c))) return rc;
+  if ((rc = mixed_state(c, form == MPIC_FORM_MIXED))) return rc;
   c->cfg.form = form;
   return MPIC_OK;
 }
@@ -539,6 +527,12 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
   c->acc_deposited = false;
   c->ke_max_prev = 0;
   HIPCHK(c, hipMemset(c->acc, 0, (size_t)c->m.ne * 16 * sizeof(unsigned long long)));
+  c->state32 = false;
+  if (c->cfg.form == MPIC_FORM_MIXED) {
+    int rc = mixed_prepare(c);
+    if (rc) return rc;
+    c->state32 = true;
+  }
   return MPIC_OK;
 }
 
@@ -568,8 +562,9 @@ int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
   HIPCHK(c, hipMemcpy(P.id, ids.data(), n * 4, hipMemcpyHostToDevice));
   c->n = n;
   c->ke_max_prev = 0;
-  int rc = invalidate_deposit(c);
-  if (rc) return rc;
+  int rc;
+  if (c->state32 && (rc = mixed_import(c, 0, n))) return rc;
+  if ((rc = invalidate_deposit(c))) return rc;
   return update_charge_scale(c);
 }
 
@@ -577,6 +572,7 @@ int mpic_particles_download(mpic_ctx* c, mpic_soa_host* h, int64_t* n) {
   if (!c || !h) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: bad arguments");
   if (materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (h->stride < c->n) return fail(c, MPIC_ERR_ARG, "mpic_particles_download: stride below the particle count");
+  if (c->state32 && mixed_export(c) != MPIC_OK) return MPIC_ERR_HIP;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevParticles& P = c->P[c->cur];
   const int64_t cnt = c->n;
@@ -623,10 +619,11 @@ int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel
   a.mass = c->cfg.species_m[type];
   a.seed_offset = seed_offset;
   HIPCHK(c, launch_fill(a, c->stream));
+  int rc;
+  if (c->state32 && (rc = mixed_import(c, c->n, add))) return rc;
   c->n += add;
   c->ke_max_prev = 0;
-  int rc = invalidate_deposit(c);
-  if (rc) return rc;
+  if ((rc = invalidate_deposit(c))) return rc;
   return update_charge_scale(c);
 }
 
@@ -634,7 +631,7 @@ int mpic_weight_charge(mpic_ctx* c) {
   if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c) return MPIC_ERR_ARG;
   tstart(c, T_DEPOSIT);
-  int rc = do_weight_charge(c);
+  int rc = c->state32 ? mixed_weight_charge(c) : do_weight_charge(c);
   tstop(c, T_DEPOSIT);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -645,7 +642,8 @@ int mpic_weight_charge(mpic_ctx* c) {
 int mpic_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   if (!c || !(tol >= 0) || max_iter < 0) return fail(c, MPIC_ERR_ARG, "mpic_solve: bad arguments");
   tstart(c, T_SOLVE);
-  int rc = do_solve(c, tol, max_iter, iters, relres);
+  int rc = c->cfg.form == MPIC_FORM_MIXED ? mixed_solve(c, tol, max_iter, iters, relres)
+                                         : do_solve(c, tol, max_iter, iters, relres);
   tstop(c, T_SOLVE);
   (void)hipStreamSynchronize(c->stream);
   c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
@@ -655,6 +653,7 @@ int mpic_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres
 int mpic_weight_efield(mpic_ctx* c) {
   if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_weight_efield: no particle list");
+  if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_weight_efield: " FP64_ONLY);
   int rc = ensure_transients(c);
   if (rc) return rc;
   HIPCHK(c, launch_gather(push_args(c), c->stream));
@@ -664,6 +663,7 @@ int mpic_weight_efield(mpic_ctx* c) {
 int mpic_accel(mpic_ctx* c, double dt, double scale) {
   if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_accel: no particle list");
+  if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_accel: " FP64_ONLY);
   int rc = ensure_transients(c);
   if (rc) return rc;
   PushArgs a = push_args(c);
@@ -676,6 +676,7 @@ int mpic_accel(mpic_ctx* c, double dt, double scale) {
 int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
   if (c && materialize(c) != MPIC_OK) return MPIC_ERR_HIP;
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_move: no particle list");
+  if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_move: " FP64_ONLY);
   int rc = ensure_transients(c);
   if (rc) return rc;
   if ((rc = invalidate_deposit(c))) return rc;
@@ -687,7 +688,7 @@ int mpic_move(mpic_ctx* c, double dt, mpic_move_stats* st) {
   PushPartial r{};
   if (c->n > 0 && (rc = finish_partials(c, partials_for(c->n), &r))) return rc;
   c->ms[T_PUSH] = c->n > 0 ? telapsed(c, T_PUSH) : 0;
-  if (st) *st = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
+  if (st) *st = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections, r.relocated};
   if (r.killed > 0 && (rc = do_sort(c, nullptr))) return rc;  // cleanup_list, particle_list.cpp:70-73
   if (r.capped > 0) return fail(c, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
   return MPIC_OK;
@@ -700,6 +701,7 @@ int mpic_compute_ke(mpic_ctx* c, double* ke) {
     *ke = 0;
     return MPIC_OK;
   }
+  if (c->state32) return mixed_compute_ke(c, ke);
   const DevParticles& P = c->P[c->cur];
   PushPartial r{};
   HIPCHK(c, launch_ke(P, c->n, c->species, 0, 0, c->partial, c->stream));
@@ -722,6 +724,7 @@ int mpic_sort_by_element(mpic_ctx* c, int32_t* start_opt) {
 
 int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   if (!c || c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_time_step: no particle list");
+  if (c->state32) return mixed_time_step(c, dt, out);
   int rc;
   mpic_step_stats st{};
   // 1. charge of the current positions (PIC.cpp:250); a pending ranking does not
@@ -752,7 +755,7 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     a.n = c->perm_n;
   }
   if (rank_after) {
-    HIPCHK(c, launch_zero(c->sb.count, (size_t)(((int64_t)c->m.ne + 2) / 2) * 8, c->stream));
+    HIPCHK(c, launch_zero(c->sb.count, (size_t)(((int64_t)c->m.nbuckets + 2) / 2) * 8, c->stream));
     a.count = c->sb.count;
   }
   tstart(c, T_PUSH);
@@ -789,15 +792,15 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   }
   tstop(c, T_KE);
   c->ke_max_prev = mx;
-  st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections};
+  st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections, r.relocated};
   c->steps++;
-  // 4. ranking by element of the new positions (counting sort; the counts came
-  //    from the push), also whenever particles were killed (cleanup_list)
+  // 4. reordering of the new positions by sort bucket (counting sort; the counts
+  //    came from the push), also whenever particles were killed (cleanup_list)
   const int64_t n_live = c->n - r.killed;
   if (rank_after) {
-    if ((rc = rank_now(c, true, n_live))) return rc;
+    if ((rc = rank_now(c, true, true, n_live))) return rc;
   } else if (r.killed > 0) {
-    if ((rc = rank_now(c, false, n_live))) return rc;
+    if ((rc = rank_now(c, true, false, n_live))) return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
@@ -843,7 +846,8 @@ int mpic_set_phi(mpic_ctx* c, const double* phi) {
 }
 
 int mpic_get_efield(mpic_ctx* c, double* E, int64_t stride) {
-  if (!c || !E || stride < c->n || !c->E) return fail(c, MPIC_ERR_STATE, "mpic_get_efield: no field gathered");
+  if (!c || !E || stride < c->n || !c->E || c->state32)
+    return fail(c, MPIC_ERR_STATE, "mpic_get_efield: no field gathered");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int d = 0; d < 3; ++d)
     HIPCHK(c, hipMemcpy(E + d * stride, c->E + d * c->cap, c->n * 8, hipMemcpyDeviceToHost));
@@ -852,8 +856,21 @@ int mpic_get_efield(mpic_ctx* c, double* E, int64_t stride) {
 
 int mpic_charge_scale(mpic_ctx* c, int* bits) {
   if (!c || !bits) return MPIC_ERR_ARG;
-  *bits = c->S;
+  *bits = c->state32 ? c->S64 : c->S;
   return MPIC_OK;
+}
+
+int mpic_check_ownership(mpic_ctx* c, int64_t* mismatches) {
+  if (!c || !mismatches) return MPIC_ERR_ARG;
+  *mismatches = 0;
+  if (c->cap == 0 || c->n == 0) return MPIC_OK;
+  if (!c->bad) {
+    int rc;
+    if ((rc = dalloc(c, &c->bad, 1))) return rc;
+    if (!c->bad_h && hipHostMalloc((void**)&c->bad_h, sizeof(unsigned long long)) != hipSuccess)
+      return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+  }
+  return mixed_check_owner(c, mismatches);
 }
 
 int mpic_kernel_time(mpic_ctx* c, int which, double* ms) {

@@ -1,0 +1,327 @@
+// Precision form (MPIC_FORM_MIXED) of the library's host side: FP32 particle
+// sets, the 64-bit fixed-point deposit, the FP32-in-FP64 refinement solve and the
+// fused time step over them (PIC::time_step, PIC.cpp:245-267).  The FP64 particle
+// set stays allocated as the staging area of fills, uploads and downloads.
+#include <algorithm>
+#include <cmath>
+
+#include "mpic_ctx.hpp"
+
+namespace mpic {
+
+int update_charge_scale(mpic_ctx* c);
+int invalidate_deposit(mpic_ctx* c);
+
+namespace {
+
+// Smallest relative residual asked of one FP32 inner solve.
+constexpr double IR_IN_TOL_MIN = 1e-5;
+
+int alloc_particles32(mpic_ctx* c) {
+  if (c->cap == 0 || c->pmem32[0]) return MPIC_OK;
+  const int64_t cap = c->cap;
+  const size_t per = sizeof(double) + 9 * sizeof(float) + sizeof(int32_t) + sizeof(uint32_t) + 1;
+  for (int k = 0; k < 2; ++k) {
+    char* base = nullptr;
+    HIPCHK(c, hipMalloc((void**)&base, per * (size_t)cap + 1024));
+    c->pmem32[k] = base;
+    DevParticles32& P = c->P32[k];
+    P.cap = cap;
+    P.w = (double*)base;
+    P.x = (float*)(P.w + cap);
+    P.xr = P.x + 3 * cap;
+    P.v = P.xr + 3 * cap;
+    P.elem = (int32_t*)(P.v + 3 * cap);
+    P.id = (uint32_t*)(P.elem + cap);
+    P.type = (uint8_t*)(P.id + cap);
+  }
+  c->cur32 = 0;
+  return MPIC_OK;
+}
+
+void record_times(mpic_ctx* c) {
+  c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
+  c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
+  c->ms[T_PUSH] = telapsed(c, T_PUSH);
+  c->ms[T_PUSH_KERNEL] = telapsed(c, T_PUSH_KERNEL);
+  c->ms[T_KE] = telapsed(c, T_KE);
+  if (c->sort_timed) {
+    c->ms[T_SORT] = telapsed(c, T_SORT);
+    c->sort_timed = false;
+  }
+}
+
+}  // namespace
+
+int mixed_prepare(mpic_ctx* c) {
+  if (!c->mixed_ready) {
+    const int32_t nn = c->m.nn, ne = c->m.ne;
+    float* geo32;
+    float* ell32;
+    float4* E32;
+    int rc;
+    if ((rc = dalloc(c, &geo32, (size_t)ne * 12)) || (rc = dalloc(c, &ell32, (size_t)c->ell_entries)) ||
+        (rc = dalloc(c, &E32, (size_t)ne)) || (rc = dalloc(c, &c->acc64, (size_t)ne)) ||
+        (rc = dalloc(c, &c->ir.r, nn)) || (rc = dalloc(c, &c->ir.d, nn)) || (rc = dalloc(c, &c->ir.r32, nn)) ||
+        (rc = dalloc(c, &c->ir.p0, nn)) ||
+        (rc = dalloc(c, &c->ir.p1, nn)) || (rc = dalloc(c, &c->ir.q, nn)) ||
+        (rc = dalloc(c, &c->ir.part_a, (size_t)2 * c->cg.nchunks)) ||
+        (rc = dalloc(c, &c->ir.part_b, (size_t)2 * c->cg.nchunks)) || (rc = dalloc(c, &c->ir.st, 1)) ||
+        (rc = dalloc(c, &c->bad, 1)))
+      return rc;
+    if (!c->background.empty() && (rc = dupload(c, &c->bg64, c->background.data(), c->background.size()))) return rc;
+    HIPCHK(c, launch_to_float(c->m.geo, geo32, (size_t)ne * 12, c->stream));
+    HIPCHK(c, launch_to_float(c->m.ell_val, ell32, (size_t)c->ell_entries, c->stream));
+    HIPCHK(c, launch_zero(c->acc64, (size_t)ne * 8, c->stream));
+    HIPCHK(c, launch_zero(c->ir.st, sizeof(IRState), c->stream));
+    if (!c->ir_h && hipHostMalloc((void**)&c->ir_h, sizeof(IRState)) != hipSuccess)
+      return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+    if (!c->bad_h && hipHostMalloc((void**)&c->bad_h, sizeof(unsigned long long)) != hipSuccess)
+      return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+    c->ir.nchunks = c->cg.nchunks;
+    c->m32.geo = geo32;
+    c->m32.E = E32;
+    c->m32.ell_val = ell32;
+    c->mixed_ready = true;
+  }
+  return alloc_particles32(c);
+}
+
+void mixed_free_particles(mpic_ctx* c) {
+  for (int k = 0; k < 2; ++k)
+    if (c->pmem32[k]) {
+      (void)hipFree(c->pmem32[k]);
+      c->pmem32[k] = nullptr;
+    }
+  c->P32[0] = c->P32[1] = DevParticles32{};
+  c->state32 = false;
+}
+
+int mixed_state(mpic_ctx* c, bool to32) {
+  if (to32 == c->state32) return MPIC_OK;
+  int rc;
+  if ((rc = materialize(c))) return rc;
+  if ((rc = invalidate_deposit(c))) return rc;
+  if (c->cap > 0) {
+    if (to32 && (rc = alloc_particles32(c))) return rc;
+    HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], 0, c->n, to32, c->stream));
+  }
+  c->state32 = to32;
+  c->ke_max_prev = 0;
+  return update_charge_scale(c);
+}
+
+int mixed_import(mpic_ctx* c, int64_t off, int64_t cnt) {
+  HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], off, cnt, true, c->stream));
+  return MPIC_OK;
+}
+
+int mixed_export(mpic_ctx* c) {
+  int rc;
+  if ((rc = materialize(c))) return rc;
+  HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], 0, c->n, false, c->stream));
+  return MPIC_OK;
+}
+
+// bound = n_live * max |w q|: element sums then stay below 2^62.
+int mixed_charge_scale(mpic_ctx* c) {
+  PushPartial r{};
+  if (c->n > 0) {
+    HIPCHK(c, launch_charge_bound32(c->P32[c->cur32], c->n, c->species, c->partial, c->stream));
+    int rc = finish_partials(c, partials_for(c->n), &r);
+    if (rc) return rc;
+  }
+  double mx;
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  c->S64 = scale_for_bound64((double)r.crossings * mx);
+  return MPIC_OK;
+}
+
+int mixed_weight_charge(mpic_ctx* c) {
+  if (!c->acc64_deposited)
+    HIPCHK(c, launch_deposit32(c->m, c->P32[c->cur32], c->n, c->species, c->S64, c->acc64, c->stream));
+  HIPCHK(c, launch_node_charge64(c->m, c->acc64, c->S64, c->bg64, c->rhs, c->stream));
+  HIPCHK(c, launch_zero(c->acc64, (size_t)c->m.ne * 8, c->stream));
+  c->acc64_deposited = false;
+  return MPIC_OK;
+}
+
+// Iterative refinement: each pass launches the FP64 residual and test, then
+// batches of FP32 inner iterations sized from the same pass of the previous
+// solve, polling the device state once per batch.
+int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
+  int rc;
+  if ((rc = mixed_prepare(c))) return rc;
+  c->ir.x = c->phi;
+  c->ir.b = c->rhs;
+  if (max_iter + 1 > c->ir_hist_cap) {
+    if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
+    c->ir.rho_hist = nullptr;
+    HIPCHK(c, hipMalloc((void**)&c->ir.rho_hist, sizeof(double) * (size_t)(max_iter + 1)));
+    c->ir_hist_cap = max_iter + 1;
+  }
+  std::vector<int> passes;
+  bool first = true;
+  for (int pass = 0;; ++pass) {
+    HIPCHK(c, launch_ir_outer(c->m, c->m32, c->ir, first, tol, max_iter, IR_IN_TOL_MIN, c->stream));
+    first = false;
+    const bool last_known = pass < (int)c->last_inner.size();
+    const bool expect_done = !c->last_inner.empty() && pass == (int)c->last_inner.size();
+    int batch = last_known ? c->last_inner[pass] + 1 : (expect_done ? 0 : 16);
+    int it = 0;
+    bool solved = false;
+    while (true) {
+      const int upto = std::min(max_iter + 1, it + batch);
+      for (; it < upto; ++it) HIPCHK(c, launch_ir_inner(c->m, c->m32, c->ir, it, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->ir_h, c->ir.st, sizeof(IRState), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (c->ir_h->done) {
+        solved = true;
+        break;
+      }
+      if (c->ir_h->in_done || it > max_iter) break;
+      batch = 16;
+    }
+    if (solved) break;
+    passes.push_back(c->ir_h->in_iters);
+    HIPCHK(c, launch_ir_update(c->m, c->ir, c->stream));
+  }
+  c->last_inner = passes;
+  const IRState& s = *c->ir_h;
+  c->last_cg_iters = s.total_inner;
+  if (iters) *iters = s.total_inner;
+  if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  if (!s.converged) return fail(c, MPIC_ERR_NOT_CONVERGED, "refinement did not converge");
+  return MPIC_OK;
+}
+
+int mixed_compute_ke(mpic_ctx* c, double* ke) {
+  int rc;
+  if ((rc = materialize(c))) return rc;
+  PushPartial r{};
+  HIPCHK(c, launch_ke32(c->P32[c->cur32], c->n, c->species, 0, 0, c->partial, c->stream));
+  if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
+  double mx;
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  const int S = scale_for_bound((double)r.crossings * mx);
+  HIPCHK(c, launch_ke32(c->P32[c->cur32], c->n, c->species, S, 1, c->partial, c->stream));
+  if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
+  *ke = from_fixed(fx128{r.ke_lo, r.ke_hi}, S);
+  c->ke_max_prev = mx;
+  return MPIC_OK;
+}
+
+int mixed_check_owner(mpic_ctx* c, int64_t* bad) {
+  int rc;
+  if ((rc = materialize(c))) return rc;
+  HIPCHK(c, launch_zero(c->bad, 8, c->stream));
+  if (c->state32)
+    HIPCHK(c, launch_check_owner32(c->m, c->P32[c->cur32], c->n, c->bad, c->stream));
+  else
+    HIPCHK(c, launch_check_owner(c->m, c->P[c->cur], c->n, c->bad, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->bad_h, c->bad, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *bad = (int64_t)*c->bad_h;
+  return MPIC_OK;
+}
+
+int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
+  int rc;
+  mpic_step_stats st{};
+  // 1. charge of the current positions
+  tstart(c, T_DEPOSIT);
+  if ((rc = mixed_weight_charge(c))) return rc;
+  tstop(c, T_DEPOSIT);
+  // 2. refinement solve, warm start from the previous potential
+  tstart(c, T_SOLVE);
+  rc = mixed_solve(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres);
+  tstop(c, T_SOLVE);
+  if (rc) return rc;
+  // 3. per-tet field, then the fused FP32 kick + walk + FP64 location + deposit + KE
+  if (!(c->ke_max_prev > 0) && c->n > 0) {
+    PushPartial r{};
+    HIPCHK(c, launch_ke32(c->P32[c->cur32], c->n, c->species, 0, 0, c->partial, c->stream));
+    if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
+    std::memcpy(&c->ke_max_prev, &r.ke_max_bits, 8);
+  }
+  const bool rank_after = c->cfg.sort_period > 0 && (c->steps + 1) % c->cfg.sort_period == 0;
+  PushArgs32 a{};
+  a.m = c->m;
+  a.m32 = c->m32;
+  a.P = c->P32[c->cur32];
+  a.Q = a.P;
+  a.n = c->n;
+  a.dt = (float)dt;
+  a.scale = 1.0f;
+  a.kill = c->cfg.kill_on_wall;
+  a.S_charge = c->S64;
+  a.S_ke = scale_for_bound((double)(c->perm_pending ? c->perm_n : c->n) * c->ke_max_prev * 256.0);
+  a.acc = c->acc64;
+  a.partial = c->partial;
+  a.species = c->species;
+  if (c->perm_pending) {
+    a.perm = c->perm;
+    a.Q = c->P32[1 - c->cur32];
+    a.n = c->perm_n;
+  }
+  if (rank_after) {
+    HIPCHK(c, launch_zero(c->sb.count, (size_t)(((int64_t)c->m.nbuckets + 2) / 2) * 8, c->stream));
+    a.count = c->sb.count;
+  }
+  tstart(c, T_PUSH);
+  HIPCHK(c, launch_field32(c->m, c->phi, c->m32.E, c->stream));
+  tstart(c, T_PUSH_KERNEL);
+  HIPCHK(c, launch_push32(a, c->stream));
+  tstop(c, T_PUSH_KERNEL);
+  if (c->perm_pending) {
+    c->cur32 = 1 - c->cur32;
+    c->n = c->perm_n;
+    c->perm_pending = false;
+  }
+  PushPartial r{};
+  if (c->n > 0 && (rc = finish_partials(c, partials_for(c->n), &r))) return rc;
+  tstop(c, T_PUSH);
+  c->acc64_deposited = true;
+  if (c->sort_timed) {
+    c->ms[T_SORT] = telapsed(c, T_SORT);
+    c->sort_timed = false;
+  }
+  double mx;
+  std::memcpy(&mx, &r.ke_max_bits, 8);
+  tstart(c, T_KE);
+  const int S_safe = scale_for_bound((double)c->n * mx);
+  if (a.S_ke > S_safe) {
+    PushPartial k{};
+    HIPCHK(c, launch_ke32(c->P32[c->cur32], c->n, c->species, S_safe, 1, c->partial, c->stream));
+    if ((rc = finish_partials(c, partials_for(c->n), &k))) return rc;
+    st.kinetic_energy = from_fixed(fx128{k.ke_lo, k.ke_hi}, S_safe);
+  } else {
+    st.kinetic_energy = from_fixed(fx128{r.ke_lo, r.ke_hi}, a.S_ke);
+  }
+  tstop(c, T_KE);
+  c->ke_max_prev = mx;
+  st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections, r.relocated};
+  c->steps++;
+  // 4. reordering by sort bucket (counts from the push), also after kills
+  const int64_t n_live = c->n - r.killed;
+  if (rank_after) {
+    if ((rc = rank_now(c, true, true, n_live))) return rc;
+  } else if (r.killed > 0) {
+    if ((rc = rank_now(c, true, false, n_live))) return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  record_times(c);
+  st.num_particles = c->perm_pending ? c->perm_n : c->n;
+  st.charge_scale_bits = c->S64;
+  st.ms_deposit = c->ms[T_DEPOSIT];
+  st.ms_solve = c->ms[T_SOLVE];
+  st.ms_push = c->ms[T_PUSH];
+  st.ms_sort = c->ms[T_SORT];
+  st.ms_ke = c->ms[T_KE];
+  if (out) *out = st;
+  // A particle stopped by the crossing cap is still placed by the FP64 location
+  // test; it is reported in the move statistics, not as an error.
+  return MPIC_OK;
+}
+
+}  // namespace mpic

@@ -1,0 +1,533 @@
+// Precision form of the particle phases of PIC::time_step (PIC.cpp:245-267):
+//   field  = the P1 field of the FP64 potential, constant on each tet
+//            (E_e = sum_i phi_i G[e][i]; the reference's gather, PIC.cpp:168-183,
+//            weights the same products by barycentric weights that sum to 1),
+//            computed in FP64 once per step and stored as float4;
+//   push   = Boris kick (B = 0) and barycentric tet walk in FP32
+//            (particle_move.hpp:136-225,364-392), then an FP64 point-location test
+//            for every particle that ends within LOC_EPS of a face of its tet,
+//            then the charge deposit of the new position and the KE;
+//   deposit = one 64-bit fixed-point integer per element: a particle adds
+//            q w (at scale 2^S) to its element, and each node of the element
+//            later receives a quarter of the element sum (every node of a tet gets
+//            -q w/4 in the reference, PIC.cpp:121-131 with
+//            sum_c wbv(e,i,c) = detJ/24).  Integer sums are exact, so node
+//            charges do not depend on the order of the atomics.
+#include "mpic_internal.hpp"
+#include "mpic_device.hpp"
+
+namespace mpic {
+namespace {
+
+// Below this minimum barycentric coordinate the FP64 test decides the owner.
+// The FP32 position carries ~3e-8 absolute error, i.e. a few 1e-6 in reference
+// coordinates of a 1/64 tet, so the band is set well above that.
+constexpr float LOC_EPS = 2e-4f;
+constexpr int LOC_CAP = 16;        // element moves of one point location
+constexpr double OWNER_TOL = 1e-9; // ownership check: tolerated negative coordinate
+
+__device__ __forceinline__ void load12f(const float* __restrict__ base, int e, float o[12]) {
+  const float4* q = reinterpret_cast<const float4*>(base + (size_t)e * 12);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float4 t = q[i];
+    o[4 * i + 0] = t.x;
+    o[4 * i + 1] = t.y;
+    o[4 * i + 2] = t.z;
+    o[4 * i + 3] = t.w;
+  }
+}
+
+// FP64 point location of the FP32 position X (widened exactly), starting in e:
+// moves across the face of the most negative barycentric coordinate until none
+// is negative, a wall is reached, the walk would step back, or LOC_CAP moves.
+// R becomes the reference position in the final element.  Face of each
+// coordinate: lambda_0 = 1 - sum R -> face 1, R0 -> face 2, R1 -> face 0, R2 -> face 3
+// (faces f0 = (0,1,3), f1 = (1,2,3), f2 = (0,3,2), f3 = (0,2,1)).
+__device__ __noinline__ void locate64(const DevMesh& m, int& e, const float X[3], float R[3], int& relocated) {
+  const double x[3] = {(double)X[0], (double)X[1], (double)X[2]};
+  double r[3];
+  int prev = -1;
+  for (int k = 0;; ++k) {
+    double g[12];
+    load12(m.geo, e, g);
+    const double dl[3] = {x[0] - g[9], x[1] - g[10], x[2] - g[11]};
+    p2r_vec(g, dl, r);
+    const double lam[4] = {1.0 - r[0] - r[1] - r[2], r[0], r[1], r[2]};
+    int j = 0;
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+      if (lam[i] < lam[j]) j = i;
+    if (!(lam[j] < 0.0) || k == LOC_CAP) break;
+    const int4 nb = m.nbrs[e];
+    const int e2 = j == 0 ? nb.y : j == 1 ? nb.z : j == 2 ? nb.x : nb.w;
+    if (e2 < 0 || e2 == prev) break;
+    prev = e;
+    e = e2;
+    relocated++;
+  }
+  R[0] = (float)r[0];
+  R[1] = (float)r[1];
+  R[2] = (float)r[2];
+}
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ void p2r_vecf(const float* Ji, const float v[3], float o[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o[r] = Ji[r * 3 + 0] * v[0] + Ji[r * 3 + 1] * v[1] + Ji[r * 3 + 2] * v[2];
+}
+
+__device__ __forceinline__ bool near_face(const float R[3]) {
+  const float l0 = 1.0f - R[0] - R[1] - R[2];
+  return fminf(fminf(l0, R[0]), fminf(R[1], R[2])) < LOC_EPS;
+}
+
+// The tet walk of MoveParticle (particle_move.hpp:103-124,136-225) in FP32.
+__device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict__ geo, int& e, float X[3], float R[3],
+                                       float V[3], float VR[3], float dt, int kill, WalkCount& wc) {
+  float tr = 1.0f;
+  int iters = 0;
+  while (tr > 0.0f) {
+    if (iters++ >= WALK_CAP) {
+      wc.capped++;
+      break;
+    }
+    const float ldt = dt * tr;
+    float s = 1.0f;
+    int face = -1;
+    float dxr[3], xf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      dxr[d] = ldt * VR[d];
+      xf[d] = R[d] + dxr[d];
+    }
+    if (VR[2] < 0 && xf[2] < 0) {
+      face = 3;
+      s = -R[2] / dxr[2];
+    }
+    if (VR[1] < 0 && xf[1] < 0) {
+      const float sfy = -R[1] / dxr[1];
+      if (sfy < s) {
+        face = 0;
+        s = sfy;
+      }
+    }
+    if (VR[0] < 0 && xf[0] < 0) {
+      const float sfx = -R[0] / dxr[0];
+      if (sfx < s) {
+        face = 2;
+        s = sfx;
+      }
+    }
+    if (VR[0] + VR[1] + VR[2] > 0 && xf[0] + xf[1] + xf[2] > 1) {
+      const float sfs = (1 - R[0] - R[1] - R[2]) / (dxr[0] + dxr[1] + dxr[2]);
+      if (sfs < s) {
+        face = 1;
+        s = sfs;
+      }
+    }
+    const float sl = s * ldt;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      X[d] += sl * V[d];
+      R[d] += sl * VR[d];
+    }
+    if (s < 1) {
+      const int4 nb = m.nbrs[e];
+      const int e2 = face == 0 ? nb.x : face == 1 ? nb.y : face == 2 ? nb.z : nb.w;
+      float g[12];
+      if (e2 >= 0) {
+        e = e2;
+        wc.crossings++;
+        load12f(geo, e, g);
+        const float dl[3] = {X[0] - g[9], X[1] - g[10], X[2] - g[11]};
+        p2r_vecf(g, dl, R);
+      } else {
+        if (kill) return false;
+        wc.reflections++;
+        const double* nrm = m.normals + (size_t)e * 12 + face * 3;
+        const float n0 = (float)nrm[0], n1 = (float)nrm[1], n2 = (float)nrm[2];
+        const float vn = V[0] * n0 + V[1] * n1 + V[2] * n2;
+        V[0] -= 2 * vn * n0;
+        V[1] -= 2 * vn * n1;
+        V[2] -= 2 * vn * n2;
+        load12f(geo, e, g);
+      }
+      p2r_vecf(g, V, VR);
+    }
+    if (!(s > 0)) wc.nonpos++;
+    tr -= s * tr;
+  }
+  return true;
+}
+
+// ---- element charge table (LDS), 64-bit fixed point --------------------------
+constexpr int DEP64 = 1024;  // entries (power of two)
+constexpr int DEP64_BITS = 10;
+
+struct DepTable64 {
+  int key[DEP64];
+  int cnt[DEP64];
+  unsigned long long val[DEP64];
+};
+
+__device__ __forceinline__ void dep64_clear(DepTable64& t) {
+  for (int k = threadIdx.x; k < DEP64; k += BLOCK) {
+    t.key[k] = DEP_EMPTY;
+    t.cnt[k] = 0;
+    t.val[k] = 0;
+  }
+}
+
+__device__ __forceinline__ void dep64_add(DepTable64& t, unsigned long long* __restrict__ acc, int* __restrict__ count,
+                                          const int32_t* __restrict__ ckey, int key, long long c) {
+  if (key < 0) return;
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - DEP64_BITS);
+  for (int probe = 0; probe < DEP64; ++probe) {
+    int k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == DEP_EMPTY) {
+      int expected = DEP_EMPTY;
+      __hip_atomic_compare_exchange_strong(&t.key[h], &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      k = expected == DEP_EMPTY ? key : expected;
+    }
+    if (k == key) {
+      atomicAdd(&t.val[h], (unsigned long long)c);
+      atomicAdd(&t.cnt[h], 1);
+      return;
+    }
+    h = (h + 1) & (DEP64 - 1);
+  }
+  __hip_atomic_fetch_add(acc + key, (unsigned long long)c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (count) atomicAdd(count + ckey[key], 1);
+}
+
+__device__ __forceinline__ void dep64_flush(DepTable64& t, unsigned long long* __restrict__ acc, int* __restrict__ count,
+                                            const int32_t* __restrict__ ckey) {
+  __syncthreads();
+  for (int k = threadIdx.x; k < DEP64; k += BLOCK) {
+    const int key = t.key[k];
+    if (key != DEP_EMPTY) {
+      const unsigned long long v = t.val[k];
+      if (v) __hip_atomic_fetch_add(acc + key, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (count) atomicAdd(count + ckey[key], t.cnt[k]);
+    }
+  }
+  __syncthreads();
+  dep64_clear(t);
+  __syncthreads();
+}
+
+// ---- kernels -----------------------------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __restrict__ phi, float4* __restrict__ E) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= m.ne) return;
+  const int4 nd = m.conn[e];
+  const double f[4] = {phi[nd.x], phi[nd.y], phi[nd.z], phi[nd.w]};
+  double G[12];
+  load12(m.grad, e, G);
+  double E0 = 0, E1 = 0, E2 = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    E0 += f[i] * G[i * 3 + 0];
+    E1 += f[i] * G[i * 3 + 1];
+    E2 += f[i] * G[i * 3 + 2];
+  }
+  E[e] = make_float4((float)E0, (float)E1, (float)E2, 0.0f);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
+  __shared__ DepTable64 table;
+  dep64_clear(table);
+  const int64_t cap = a.P.cap;
+  const int64_t qcap = a.Q.cap;
+  const bool gather = a.perm != nullptr;
+  WalkCount wc{};
+  long long killed = 0;
+  fx128 ke{0, 0};
+  unsigned long long kmax = 0;
+  int64_t lo, hi;
+  block_range(a.n, &lo, &hi);
+  __syncthreads();
+  int chunk = 0;
+  for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
+    const int64_t p = base + threadIdx.x;  // output position
+    int enew = -1;
+    double w = 0;
+    float V[3] = {0, 0, 0};
+    int t = 0;
+    if (p < hi) {
+      const int64_t src = gather ? (int64_t)a.perm[p] : p;
+      int e = a.P.elem[src];
+      if (e >= 0) {
+        float X[3], R[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          X[d] = a.P.x[d * cap + src];
+          R[d] = a.P.xr[d * cap + src];
+          V[d] = a.P.v[d * cap + src];
+        }
+        t = a.P.type[src];
+        w = a.P.w[src];
+        const float4 E = a.m32.E[e];
+        float g[12];
+        load12f(a.m32.geo, e, g);
+        // Boris push with B = 0 (two half kicks), particle_move.hpp:364-392
+        const float hq = a.dt * a.scale * (float)a.species[32 + t] * 0.5f;
+        const float Ev[3] = {E.x, E.y, E.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          V[d] += hq * Ev[d];
+          V[d] += hq * Ev[d];
+        }
+        float VR[3];
+        p2r_vecf(g, V, VR);
+        const bool alive = walk32(a.m, a.m32.geo, e, X, R, V, VR, a.dt, a.kill, wc);
+        if (alive && near_face(R)) locate64(a.m, e, X, R, wc.relocated);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a.Q.x[d * qcap + p] = X[d];
+          a.Q.xr[d * qcap + p] = R[d];
+          a.Q.v[d * qcap + p] = V[d];
+        }
+        enew = alive ? e : -1;
+        killed += alive ? 0 : 1;
+        a.Q.elem[p] = enew;
+        if (gather) {
+          a.Q.w[p] = w;
+          a.Q.type[p] = (uint8_t)t;
+          a.Q.id[p] = a.P.id[src];
+        }
+      } else if (gather) {
+        a.Q.elem[p] = -1;
+      }
+    }
+    if (enew >= 0) {
+      dep64_add(table, a.acc, a.count, a.m.bucket, enew, to_fixed64(w * a.species[t], a.S_charge));
+      const double Vd[3] = {(double)V[0], (double)V[1], (double)V[2]};
+      const double term = ke_term(w, a.species[16 + t], Vd);
+      ke = fx_add(ke, to_fixed(term, a.S_ke));
+      const unsigned long long tb = __double_as_longlong(term);
+      kmax = tb > kmax ? tb : kmax;
+    }
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep64_flush(table, a.acc, a.count, a.m.bucket);
+  }
+  dep64_flush(table, a.acc, a.count, a.m.bucket);
+  block_partial(a.partial, ke, kmax, wc, killed);
+}
+
+// Deposit of the current positions (weight_charge, PIC.cpp:139-160).
+__global__ __launch_bounds__(BLOCK) void k_deposit32(DevMesh m, DevParticles32 P, int64_t n, const double* sp, int S,
+                                                     unsigned long long* acc) {
+  __shared__ DepTable64 table;
+  dep64_clear(table);
+  int64_t lo, hi;
+  block_range(n, &lo, &hi);
+  __syncthreads();
+  int chunk = 0;
+  for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
+    const int64_t p = base + threadIdx.x;
+    if (p < hi) {
+      const int e = P.elem[p];
+      if (e >= 0) dep64_add(table, acc, nullptr, nullptr, e, to_fixed64(P.w[p] * sp[P.type[p]], S));
+    }
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep64_flush(table, acc, nullptr, nullptr);
+  }
+  dep64_flush(table, acc, nullptr, nullptr);
+}
+
+// Node charges from the element sums (the finalize half of PIC::weight_charge,
+// PIC.cpp:139-160): sum exactly in 64-bit, one rounding to FP64, a quarter per
+// node, background added, boundary rows zeroed (PIC.cpp:155-158).
+__global__ __launch_bounds__(BLOCK) void k_node_charge64(DevMesh m, const unsigned long long* __restrict__ acc, int S,
+                                                         const double* __restrict__ bg, double* __restrict__ rhs) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= m.nn) return;
+  unsigned long long s = 0;
+  const int b = m.adj_ptr[i], e = m.adj_ptr[i + 1];
+  for (int k = b; k < e; ++k) s += acc[m.adj[k] >> 2];
+  const double q = ldexp((double)(long long)s, -S);
+  rhs[i] = m.bnd[i] ? 0.0 : (bg ? bg[i] : 0.0) + -0.25 * q;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_charge_bound32(DevParticles32 P, int64_t n, const double* sp,
+                                                          PushPartial* partial) {
+  unsigned long long qmax = 0;
+  WalkCount live{};
+  for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
+    if (P.elem[p] < 0) continue;
+    const unsigned long long qb = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
+    qmax = qb > qmax ? qb : qmax;
+    live.crossings++;
+  }
+  block_partial(partial, fx128{0, 0}, qmax, live, 0);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ke32(DevParticles32 P, int64_t n, const double* sp, int S, int want_sum,
+                                                PushPartial* partial) {
+  fx128 ke{0, 0};
+  unsigned long long kmax = 0;
+  WalkCount live{};
+  for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
+    if (P.elem[p] < 0) continue;
+    double V[3];
+    for (int d = 0; d < 3; ++d) V[d] = (double)P.v[d * P.cap + p];
+    const double term = ke_term(P.w[p], sp[16 + P.type[p]], V);
+    if (want_sum) ke = fx_add(ke, to_fixed(term, S));
+    const unsigned long long tb = __double_as_longlong(term);
+    kmax = tb > kmax ? tb : kmax;
+    live.crossings++;
+  }
+  block_partial(partial, ke, kmax, live, 0);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_permute32(DevParticles32 src, DevParticles32 dst,
+                                                     const int32_t* __restrict__ perm, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const int64_t p = perm[q];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dst.x[d * dst.cap + q] = src.x[d * src.cap + p];
+    dst.xr[d * dst.cap + q] = src.xr[d * src.cap + p];
+    dst.v[d * dst.cap + q] = src.v[d * src.cap + p];
+  }
+  dst.w[q] = src.w[p];
+  dst.elem[q] = src.elem[p];
+  dst.type[q] = src.type[p];
+  dst.id[q] = src.id[p];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticles32 P32, int64_t off, int64_t n,
+                                                   int to32) {
+  const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = off + j;
+  if (to32) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      P32.x[d * P32.cap + i] = (float)P64.x[d * P64.cap + i];
+      P32.xr[d * P32.cap + i] = (float)P64.xr[d * P64.cap + i];
+      P32.v[d * P32.cap + i] = (float)P64.v[d * P64.cap + i];
+    }
+    P32.w[i] = P64.w[i];
+    P32.elem[i] = P64.elem[i];
+    P32.type[i] = P64.type[i];
+    P32.id[i] = P64.id[i];
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      P64.x[d * P64.cap + i] = (double)P32.x[d * P32.cap + i];
+      P64.xr[d * P64.cap + i] = (double)P32.xr[d * P32.cap + i];
+      P64.v[d * P64.cap + i] = (double)P32.v[d * P32.cap + i];
+    }
+    P64.w[i] = P32.w[i];
+    P64.elem[i] = P32.elem[i];
+    P64.type[i] = P32.type[i];
+    P64.id[i] = P32.id[i];
+  }
+}
+
+__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+// Ownership check: a live particle's element must contain its position in FP64
+// (a coordinate below -OWNER_TOL across an interior face is a mismatch; across
+// a wall face the position is outside the domain by rounding and is not counted).
+template <class PT>
+__global__ __launch_bounds__(BLOCK) void k_check_owner(DevMesh m, PT P, int64_t n, unsigned long long* bad) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  int mismatch = 0;
+  if (p < n) {
+    const int e = P.elem[p];
+    if (e >= 0) {
+      double g[12];
+      load12(m.geo, e, g);
+      const double dl[3] = {(double)P.x[p] - g[9], (double)P.x[P.cap + p] - g[10], (double)P.x[2 * P.cap + p] - g[11]};
+      double r[3];
+      p2r_vec(g, dl, r);
+      const double lam[4] = {1.0 - r[0] - r[1] - r[2], r[0], r[1], r[2]};
+      const int4 nb = m.nbrs[e];
+      const int across[4] = {nb.y, nb.z, nb.x, nb.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (lam[j] < -OWNER_TOL && across[j] >= 0) mismatch = 1;
+    }
+  }
+  const int cnt = __syncthreads_count(mismatch);
+  if (threadIdx.x == 0 && cnt) atomicAdd(bad, (unsigned long long)cnt);
+}
+
+}  // namespace
+
+hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStream_t s) {
+  hipLaunchKernelGGL(k_field32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, phi, E);
+  return hipGetLastError();
+}
+hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  if (!a.m32.geo || !a.m32.E || !a.P.x || !a.Q.x || !a.acc || !a.partial || !a.species || !a.m.bucket)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_push32, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_deposit32(const DevMesh& m, const DevParticles32& P, int64_t n, const double* species, int S,
+                            unsigned long long* acc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_deposit32, dim3(grid_stride(n)), dim3(BLOCK), 0, s, m, P, n, species, S, acc);
+  return hipGetLastError();
+}
+hipError_t launch_node_charge64(const DevMesh& m, const unsigned long long* acc, int S, const double* background,
+                                double* rhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_node_charge64, dim3(grid_for(m.nn)), dim3(BLOCK), 0, s, m, acc, S, background, rhs);
+  return hipGetLastError();
+}
+hipError_t launch_charge_bound32(const DevParticles32& P, int64_t n, const double* species, PushPartial* partial,
+                                 hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_charge_bound32, dim3(grid_stride(n)), dim3(BLOCK), 0, s, P, n, species, partial);
+  return hipGetLastError();
+}
+hipError_t launch_ke32(const DevParticles32& P, int64_t n, const double* species, int S, int want_sum,
+                       PushPartial* partial, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ke32, dim3(grid_stride(n)), dim3(BLOCK), 0, s, P, n, species, S, want_sum, partial);
+  return hipGetLastError();
+}
+hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst, const int32_t* perm, int64_t n,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute32, dim3(grid_for(n)), dim3(BLOCK), 0, s, src, dst, perm, n);
+  return hipGetLastError();
+}
+hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n, bool to32,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_convert, dim3(grid_for(n)), dim3(BLOCK), 0, s, P64, P32, off, n, to32 ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t g = (n + BLOCK - 1) / BLOCK;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_to_float, dim3((unsigned)g), dim3(BLOCK), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+hipError_t launch_check_owner(const DevMesh& m, const DevParticles& P, int64_t n, unsigned long long* bad,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_check_owner<DevParticles>, dim3(grid_for(n)), dim3(BLOCK), 0, s, m, P, n, bad);
+  return hipGetLastError();
+}
+hipError_t launch_check_owner32(const DevMesh& m, const DevParticles32& P, int64_t n, unsigned long long* bad,
+                                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_check_owner<DevParticles32>, dim3(grid_for(n)), dim3(BLOCK), 0, s, m, P, n, bad);
+  return hipGetLastError();
+}
+
+}  // namespace mpic

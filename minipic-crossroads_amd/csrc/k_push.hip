@@ -1,9 +1,9 @@
 // Particle kernels of the MiniPIC timestep on gfx950:
 //   fused push  = E gather + Boris kick + barycentric tet walk + charge deposit of
-//                 the new position + kinetic energy + element counts of the new
-//                 positions, one thread per particle; when a permutation from the
-//                 previous step's ranking is given, each thread gathers its
-//                 particle through it and writes the result in element order;
+//                 the new position + kinetic energy + sort-bucket counts of the
+//                 new positions, one thread per particle; when a permutation from
+//                 the previous step's ranking is given, each thread gathers its
+//                 particle through it and writes the result in that order;
 //   split phases = the same arithmetic as separate launches (weight_Efield,
 //                 accel, move) for the reference's phase-by-phase API.
 // All FP64 expressions keep the reference's association order and are compiled
@@ -58,9 +58,6 @@ __device__ __forceinline__ void kick(const double* Ji, double V[3], const double
   p2r_vec(Ji, V, VR);
 }
 
-struct WalkCount {
-  int crossings, capped, nonpos, reflections;
-};
 
 // MoveParticle::operator() + find_intersection, particle_move.hpp:103-124,136-225.
 // Returns false when the particle is killed on a wall.
@@ -167,56 +164,6 @@ __device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double 
   }
 }
 
-__device__ __forceinline__ double ke_term(double w, double mass, const double V[3]) {
-  const double tm = w * mass;
-  double vs = 0;
-  vs += V[0] * V[0];
-  vs += V[1] * V[1];
-  vs += V[2] * V[2];
-  return 0.5 * tm * vs;
-}
-
-// Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x].
-__device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsigned long long kmax, const WalkCount& wc,
-                                              long long killed) {
-  __shared__ PushPartial sh[BLOCK / 64];
-  unsigned long long klo = ke.lo;
-  long long khi = ke.hi;
-  long long c0 = wc.crossings, c1 = killed, c2 = wc.capped, c3 = wc.nonpos, c4 = wc.reflections;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    unsigned long long olo = __shfl_xor(klo, off);
-    long long ohi = __shfl_xor(khi, off);
-    unsigned long long nlo = klo + olo;
-    khi = khi + ohi + (nlo < klo ? 1 : 0);
-    klo = nlo;
-    unsigned long long om = __shfl_xor(kmax, off);
-    kmax = om > kmax ? om : kmax;
-    c0 += __shfl_xor(c0, off);
-    c1 += __shfl_xor(c1, off);
-    c2 += __shfl_xor(c2, off);
-    c3 += __shfl_xor(c3, off);
-    c4 += __shfl_xor(c4, off);
-  }
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) sh[wave] = PushPartial{klo, khi, kmax, c0, c1, c2, c3, c4};
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    PushPartial r = sh[0];
-    for (int w = 1; w < BLOCK / 64; ++w) {
-      fx128 t = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
-      r.ke_lo = t.lo;
-      r.ke_hi = t.hi;
-      r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
-      r.crossings += sh[w].crossings;
-      r.killed += sh[w].killed;
-      r.capped += sh[w].capped;
-      r.nonpos += sh[w].nonpos;
-      r.reflections += sh[w].reflections;
-    }
-    out[blockIdx.x] = r;
-  }
-}
 
 struct KArgs {
   DevMesh m;
@@ -235,26 +182,13 @@ struct KArgs {
   const double* sp;  // [48]: q[16], m[16], q/m[16]
 };
 
-// Each workgroup handles one contiguous range of particles in chunks of BLOCK
-// (a wavefront covers 64 consecutive particles); element charges of the new
-// positions are accumulated in an LDS table and flushed every FLUSH_CHUNKS.
-constexpr int FLUSH_CHUNKS = 4;
-
-__device__ __forceinline__ void block_range(int64_t n, int64_t* lo, int64_t* hi) {
-  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
-  const int64_t per = (chunks + gridDim.x - 1) / gridDim.x;
-  *lo = (int64_t)blockIdx.x * per * BLOCK;
-  const int64_t h = *lo + per * BLOCK;
-  *hi = h < n ? h : n;
-}
-
 __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   __shared__ DepTable table;
   dep_table_clear(table);
   const int64_t cap = a.P.cap;
   const int64_t qcap = a.Q.cap;
   const bool gather = a.perm != nullptr;
-  WalkCount wc{0, 0, 0, 0};
+  WalkCount wc{};
   long long killed = 0;
   fx128 ke{0, 0};
   unsigned long long kmax = 0;
@@ -308,15 +242,15 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     if (enew >= 0) {
       fx128 c[4];
       particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
-      dep_table_add(table, a.acc, a.count, enew, c);
+      dep_table_add(table, a.acc, a.count, a.m.bucket, enew, c);
       const double term = ke_term(w, a.sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
       const unsigned long long tb = __double_as_longlong(term);
       kmax = tb > kmax ? tb : kmax;
     }
-    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, a.acc, a.count);
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, a.acc, a.count, a.m.bucket);
   }
-  dep_table_flush(table, a.acc, a.count);
+  dep_table_flush(table, a.acc, a.count, a.m.bucket);
   block_partial(a.partial, ke, kmax, wc, killed);
 }
 
@@ -357,7 +291,7 @@ __global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
 // MoveParticle: the tet walk from the stored v and v_ref (no deposit).
 __global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
   const int64_t cap = a.P.cap;
-  WalkCount wc{0, 0, 0, 0};
+  WalkCount wc{};
   long long killed = 0;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < a.n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t p = base + threadIdx.x;
@@ -399,12 +333,12 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, in
       if (e >= 0) {
         fx128 c[4];
         particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
-        dep_table_add(table, acc, nullptr, e, c);
+        dep_table_add(table, acc, nullptr, nullptr, e, c);
       }
     }
-    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, acc, nullptr);
+    if ((chunk + 1) % FLUSH_CHUNKS == 0) dep_table_flush(table, acc, nullptr, nullptr);
   }
-  dep_table_flush(table, acc, nullptr);
+  dep_table_flush(table, acc, nullptr, nullptr);
 }
 
 // KE terms (PIC.cpp:191-200): the per-workgroup max term, the live count (in
@@ -413,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_ke(DevParticles P, int64_t n, const d
                                               PushPartial* partial) {
   fx128 ke{0, 0};
   unsigned long long kmax = 0;
-  WalkCount live{0, 0, 0, 0};
+  WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
     if (P.elem[p] < 0) continue;
     double V[3];
@@ -432,7 +366,7 @@ __global__ __launch_bounds__(BLOCK) void k_ke(DevParticles P, int64_t n, const d
 __global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t n, const double* sp,
                                                         PushPartial* partial) {
   unsigned long long qmax = 0;
-  WalkCount live{0, 0, 0, 0};
+  WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
     if (P.elem[p] < 0) continue;
     const unsigned long long qb = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
@@ -446,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t 
 __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, int nparts, PushPartial* out) {
   fx128 ke{0, 0};
   unsigned long long mx = 0;
-  long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
   for (int i = threadIdx.x; i < nparts; i += BLOCK) {
     const PushPartial q = part[i];
     ke = fx_add(ke, fx128{q.ke_lo, q.ke_hi});
@@ -456,9 +390,10 @@ __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, 
     c2 += q.capped;
     c3 += q.nonpos;
     c4 += q.reflections;
+    c5 += q.relocated;
   }
   __shared__ PushPartial sh[BLOCK];
-  sh[threadIdx.x] = PushPartial{ke.lo, ke.hi, mx, c0, c1, c2, c3, c4};
+  sh[threadIdx.x] = PushPartial{ke.lo, ke.hi, mx, c0, c1, c2, c3, c4, c5};
   __syncthreads();
   if (threadIdx.x == 0) {
     PushPartial r = sh[0];
@@ -472,6 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, 
       r.capped += sh[w].capped;
       r.nonpos += sh[w].nonpos;
       r.reflections += sh[w].reflections;
+      r.relocated += sh[w].relocated;
     }
     *out = r;
   }
@@ -480,12 +416,6 @@ __global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, 
 KArgs kargs(const PushArgs& a) {
   return KArgs{a.m, a.P, a.perm ? a.Q : a.P, a.perm, a.count, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge,
                a.S_ke, a.acc, a.partial, a.E, a.vr, a.species};
-}
-
-inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-inline unsigned grid_stride(int64_t n) {
-  const int64_t g = (n + BLOCK - 1) / BLOCK;
-  return (unsigned)(g < MAX_PARTIALS ? g : MAX_PARTIALS);
 }
 
 }  // namespace

@@ -1,16 +1,20 @@
-// Counting sort of the particle list by owning element: the build's replacement
-// for ParticleList::sort (Kokkos::BinSort, particle_list.cpp:90-116) and
-// compute_start_particles_by_element (particle_list.cpp:118-146).
-//   1. count[e] = particles in element e (key-grouped atomics; the fused push
-//      produces the counts itself);
-//   2. start = exclusive scan of count (start[ne] = live particles);
-//   3. rank: perm[start[e] + rank] = position of every live particle; dead
+// Counting sort of the particle list: the build's replacement for
+// ParticleList::sort (Kokkos::BinSort, particle_list.cpp:90-116) and
+// compute_start_particles_by_element (particle_list.cpp:118-146).  The key is the
+// owning element, or, for the timestep's own reordering, the spatial bucket of
+// the owning element (a few dozen neighbouring elements): most particles stay in
+// their bucket from one step to the next, so the reordering keeps their relative
+// order and the next push reads them almost in sequence.
+//   1. count[key] = particles with that key (key-grouped atomics; the fused push
+//      produces bucket counts itself);
+//   2. start = exclusive scan of count (start[nkeys] = live particles);
+//   3. rank: perm[start[key] + rank] = position of every live particle; dead
 //      particles (elem < 0) get no slot, which also compacts the list
 //      (DeleteParticles, particle_list.cpp:148-191).
 // Only the 4-byte permutation is scattered: the next fused push gathers each
-// particle through it and writes the whole state in element order (or
-// launch_permute applies it).  As in the reference, the order inside one element
-// is not specified (it follows the order of the per-wavefront atomics); the
+// particle through it and writes the whole state in key order (or
+// launch_permute applies it).  As in the reference, the order inside one key is
+// not specified (it follows the order of the per-wavefront atomics); the
 // identity key (id) travels with the particle.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
@@ -24,9 +28,17 @@ constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 // Lanes of a wavefront with the same element are grouped (wave_group_leader);
 // the leader adds the group size with one atomic, so every atomic instruction
 // touches distinct addresses.
-__global__ __launch_bounds__(BLOCK) void k_count(const int32_t* __restrict__ elem, int64_t n, int32_t* count) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+// Sort key of particle p: -1 when dead or out of range.
+__device__ __forceinline__ int sort_key(const int32_t* __restrict__ elem, const int32_t* __restrict__ kmap, int64_t p,
+                                        int64_t n) {
   const int e = p < n ? elem[p] : -1;
+  return (e >= 0 && kmap) ? kmap[e] : e;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_count(const int32_t* __restrict__ elem, const int32_t* __restrict__ kmap,
+                                                 int64_t n, int32_t* count) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int e = sort_key(elem, kmap, p, n);
   unsigned long long grp;
   const int leader = wave_group_leader(e, &grp);
   if (leader == (int)(threadIdx.x & 63)) atomicAdd(count + e, (int)__popcll(grp));
@@ -107,11 +119,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rank(const int32_t* __restrict__ elem, int64_t n,
-                                                int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+__global__ __launch_bounds__(BLOCK) void k_rank(const int32_t* __restrict__ elem, const int32_t* __restrict__ kmap,
+                                                int64_t n, int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int e = p < n ? elem[p] : -1;
+  const int e = sort_key(elem, kmap, p, n);
   unsigned long long grp;
   const int leader = wave_group_leader(e, &grp);
   int base = 0;
@@ -141,21 +153,23 @@ __global__ __launch_bounds__(BLOCK) void k_permute(DevParticles src, DevParticle
 
 }  // namespace
 
-hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool counts_ready, SortBuffers& sb,
-                       int32_t* perm, hipStream_t s) {
-  const int64_t ne1 = (int64_t)m.ne + 1;
+hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool by_bucket, bool counts_ready,
+                       SortBuffers& sb, int32_t* perm, hipStream_t s) {
+  const int32_t* kmap = by_bucket ? m.bucket : nullptr;
+  const int64_t nkeys = by_bucket ? m.nbuckets : m.ne;
+  const int64_t nk1 = nkeys + 1;
   const unsigned gp = (unsigned)((n + BLOCK - 1) / BLOCK);
   if (!counts_ready) {
-    hipError_t err = launch_zero(sb.count, (size_t)((ne1 + 1) / 2) * 8, s);
+    hipError_t err = launch_zero(sb.count, (size_t)((nk1 + 1) / 2) * 8, s);
     if (err != hipSuccess) return err;
-    if (n > 0) hipLaunchKernelGGL(k_count, dim3(gp), dim3(BLOCK), 0, s, P.elem, n, sb.count);
+    if (n > 0) hipLaunchKernelGGL(k_count, dim3(gp), dim3(BLOCK), 0, s, P.elem, kmap, n, sb.count);
   }
-  const int ntiles = (int)((ne1 + SCAN_TILE - 1) / SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum);
+  const int ntiles = (int)((nk1 + SCAN_TILE - 1) / SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, nk1, sb.blocksum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, sb.blocksum, ntiles);
-  hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, ne1, sb.blocksum, sb.start, sb.cursor,
-                     (int64_t)m.ne);
-  if (n > 0) hipLaunchKernelGGL(k_rank, dim3(gp), dim3(BLOCK), 0, s, P.elem, n, sb.cursor, perm);
+  hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, nk1, sb.blocksum, sb.start, sb.cursor,
+                     nkeys);
+  if (n > 0) hipLaunchKernelGGL(k_rank, dim3(gp), dim3(BLOCK), 0, s, P.elem, kmap, n, sb.cursor, perm);
   return hipGetLastError();
 }
 

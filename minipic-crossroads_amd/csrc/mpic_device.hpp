@@ -81,6 +81,78 @@ __device__ __forceinline__ void lds_add_fx(fx128* dst, fx128 v) {
   if (h != 0ull) atomicAdd(reinterpret_cast<unsigned long long*>(&dst->hi), h);
 }
 
+// Kinetic energy term of one particle, PIC.cpp:191-200: 0.5 (w m) |v|^2.
+__device__ __forceinline__ double ke_term(double w, double mass, const double V[3]) {
+  const double tm = w * mass;
+  double vs = 0;
+  vs += V[0] * V[0];
+  vs += V[1] * V[1];
+  vs += V[2] * V[2];
+  return 0.5 * tm * vs;
+}
+
+// Walk counters of one thread.
+struct WalkCount {
+  int crossings, capped, nonpos, reflections, relocated;
+};
+
+// Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x].
+__device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsigned long long kmax, const WalkCount& wc,
+                                              long long killed) {
+  __shared__ PushPartial sh[BLOCK / 64];
+  unsigned long long klo = ke.lo;
+  long long khi = ke.hi;
+  long long c0 = wc.crossings, c1 = killed, c2 = wc.capped, c3 = wc.nonpos, c4 = wc.reflections, c5 = wc.relocated;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    unsigned long long olo = __shfl_xor(klo, off);
+    long long ohi = __shfl_xor(khi, off);
+    unsigned long long nlo = klo + olo;
+    khi = khi + ohi + (nlo < klo ? 1 : 0);
+    klo = nlo;
+    unsigned long long om = __shfl_xor(kmax, off);
+    kmax = om > kmax ? om : kmax;
+    c0 += __shfl_xor(c0, off);
+    c1 += __shfl_xor(c1, off);
+    c2 += __shfl_xor(c2, off);
+    c3 += __shfl_xor(c3, off);
+    c4 += __shfl_xor(c4, off);
+    c5 += __shfl_xor(c5, off);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[wave] = PushPartial{klo, khi, kmax, c0, c1, c2, c3, c4, c5};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    PushPartial r = sh[0];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      fx128 t = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
+      r.ke_lo = t.lo;
+      r.ke_hi = t.hi;
+      r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
+      r.crossings += sh[w].crossings;
+      r.killed += sh[w].killed;
+      r.capped += sh[w].capped;
+      r.nonpos += sh[w].nonpos;
+      r.reflections += sh[w].reflections;
+      r.relocated += sh[w].relocated;
+    }
+    out[blockIdx.x] = r;
+  }
+}
+
+// Each workgroup handles one contiguous range of particles in chunks of BLOCK
+// (a wavefront covers 64 consecutive particles); element charges of the new
+// positions are accumulated in an LDS table and flushed every FLUSH_CHUNKS.
+constexpr int FLUSH_CHUNKS = 4;
+
+__device__ __forceinline__ void block_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+  const int64_t per = (chunks + gridDim.x - 1) / gridDim.x;
+  *lo = (int64_t)blockIdx.x * per * BLOCK;
+  const int64_t h = *lo + per * BLOCK;
+  *hi = h < n ? h : n;
+}
+
 // ---------------------------------------------------------------------------
 // Block-level charge accumulation.
 //
@@ -123,10 +195,11 @@ __device__ __forceinline__ unsigned long long fx_digit(fx128 v, int d) {
   }
 }
 
-// Direct global add of one particle's 4 node charges and count (table overflow path).
-__device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ acc, int* __restrict__ count, int key,
-                                               const fx128 c[4]) {
-  if (count) atomicAdd(count + key, 1);
+// Direct global add of one particle's 4 node charges and count (table overflow
+// path).  Counts are kept per ckey[element] (the sort bucket).
+__device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ acc, int* __restrict__ count,
+                                               const int32_t* __restrict__ ckey, int key, const fx128 c[4]) {
+  if (count) atomicAdd(count + ckey[key], 1);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -140,7 +213,8 @@ __device__ __forceinline__ void dep_global_add(unsigned long long* __restrict__ 
 // nothing) and counts it.  Uses LDS atomics; falls back to global atomics if the
 // table is full.
 __device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* __restrict__ acc,
-                                              int* __restrict__ count, int key, const fx128 c[4]) {
+                                              int* __restrict__ count, const int32_t* __restrict__ ckey, int key,
+                                              const fx128 c[4]) {
   if (key < 0) return;
   unsigned h = ((unsigned)key * 2654435761u) >> (32 - 9);  // 9 = log2(DEP_TABLE)
   for (int probe = 0; probe < DEP_TABLE; ++probe) {
@@ -159,13 +233,13 @@ __device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* _
     }
     h = (h + 1) & (DEP_TABLE - 1);
   }
-  dep_global_add(acc, count, key, c);
+  dep_global_add(acc, count, ckey, key, c);
 }
 
 // Flushes every used entry with coalesced non-returning atomics (charges; counts
 // when count != null) and clears the table.  Must be called by the whole workgroup.
 __device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long* __restrict__ acc,
-                                                int* __restrict__ count) {
+                                                int* __restrict__ count, const int32_t* __restrict__ ckey) {
   __syncthreads();
   const int sub = threadIdx.x & 15;  // node = sub/4, digit = sub%4
   for (int e = threadIdx.x >> 4; e < DEP_TABLE; e += BLOCK / 16) {
@@ -173,7 +247,7 @@ __device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long*
     if (key != DEP_EMPTY) {
       const unsigned long long w = fx_digit(t.val[e][sub >> 2], sub & 3);
       if (w) __hip_atomic_fetch_add(acc + (size_t)key * 16 + sub, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (count && sub == 0) atomicAdd(count + key, t.cnt[e]);
+      if (count && sub == 0) atomicAdd(count + ckey[key], t.cnt[e]);
     }
   }
   __syncthreads();

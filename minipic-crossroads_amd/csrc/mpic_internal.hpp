@@ -41,6 +41,8 @@ struct DevMesh {
   const int32_t* adj_ptr;  // [nn+1]    node -> slots e*4+k of elements touching it
   const int32_t* adj;      // [4*ne]
   const fx128* bgfix;      // [nn]      background load vector in fixed point at the current scale
+  const int32_t* bucket;   // [ne]      spatial bucket of each element (particle ordering key)
+  int32_t nbuckets;
   // Stiffness matrix in sliced ELL (64-row slices, entries column-major inside
   // a slice so a wavefront reads one entry of 64 rows per load).  Each row keeps
   // the CSR entry order; padding entries are (col = row, val = +0).
@@ -64,6 +66,26 @@ struct DevParticles {
   int64_t cap;
 };
 
+// Precision-form particle SoA: FP32 position, reference position and velocity;
+// the weight stays FP64 because charges are summed exactly from it.
+struct DevParticles32 {
+  float* x;       // [3][cap]
+  float* xr;      // [3][cap]
+  float* v;       // [3][cap]
+  double* w;      // [cap]
+  int32_t* elem;  // [cap], -1 dead
+  uint8_t* type;  // [cap]
+  uint32_t* id;   // [cap]
+  int64_t cap;
+};
+
+// Precision-form mesh tables.
+struct Mesh32 {
+  const float* geo;       // [ne][12] inverse Jacobian (row-major) then node 0, FP32
+  float4* E;              // [ne]     field of the current potential (constant per tet)
+  const float* ell_val;   // FP32 stiffness values in the sliced ELL layout of DevMesh
+};
+
 struct Species {
   double q[MPIC_MAX_SPECIES];
   double m[MPIC_MAX_SPECIES];
@@ -75,7 +97,7 @@ struct PushPartial {
   unsigned long long ke_lo;
   long long ke_hi;
   unsigned long long ke_max_bits;  // max KE term (non-negative double bits)
-  long long crossings, killed, capped, nonpos, reflections;
+  long long crossings, killed, capped, nonpos, reflections, relocated;
 };
 
 struct CGState {
@@ -124,6 +146,25 @@ inline int scale_for_bound(double bound) {
   return S;
 }
 
+// 64-bit fixed point of the precision-form deposit: scale so that any sum
+// bounded by `bound` stays below 2^62.
+inline int scale_for_bound64(double bound) {
+  if (!(bound > 0) || !std::isfinite(bound)) return 40;
+  int S = 62 - (std::ilogb(bound) + 1);
+  if (S > 1000) S = 1000;
+  if (S < -1000) S = -1000;
+  return S;
+}
+__host__ __device__ inline long long to_fixed64(double v, int S) { return (long long)rint(ldexp(v, S)); }
+
+// Launch shapes: one thread per item, or a grid-stride grid capped at MAX_PARTIALS.
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+inline unsigned grid_stride(int64_t n) {
+  const int64_t g = (n + BLOCK - 1) / BLOCK;
+  return (unsigned)(g < MAX_PARTIALS ? g : MAX_PARTIALS);
+}
+
+
 // ---------------------------------------------------------------------------
 // Launch wrappers (one .hip file each).  All are stream-ordered and return
 // hipError_t of the launch.
@@ -133,7 +174,7 @@ struct PushArgs {
   DevParticles P;          // input particles
   DevParticles Q;          // output (fused push with perm)
   const int32_t* perm;     // output position -> input position, or null (push in place)
-  int32_t* count;          // element counts of the new positions (fused push), or null
+  int32_t* count;          // bucket counts (m.bucket) of the new positions (fused push), or null
   int64_t n;
   const double* phi;
   double dt, scale;
@@ -176,6 +217,66 @@ hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_it
 // Iteration it: convergence test + p/q update, then (it < max_iter) the x/r update.
 hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s);
 
+// ---- precision form (k_mixed.hip, k_cg32.hip) ----
+struct PushArgs32 {
+  DevMesh m;
+  Mesh32 m32;
+  DevParticles32 P;        // input particles
+  DevParticles32 Q;        // output (with perm)
+  const int32_t* perm;     // output position -> input position, or null (in place)
+  int32_t* count;          // bucket counts of the new positions, or null
+  int64_t n;
+  float dt, scale;
+  int kill, S_charge, S_ke;
+  unsigned long long* acc;  // [ne] element charge sums, 64-bit fixed point at 2^-S_charge
+  PushPartial* partial;
+  const double* species;
+};
+hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStream_t s);
+hipError_t launch_push32(const PushArgs32& a, hipStream_t s);
+hipError_t launch_deposit32(const DevMesh& m, const DevParticles32& P, int64_t n, const double* species, int S,
+                            unsigned long long* acc, hipStream_t s);
+// rhs[i] = boundary ? 0 : background[i] - (sum of acc over elements of node i) * 2^-S / 4.
+hipError_t launch_node_charge64(const DevMesh& m, const unsigned long long* acc, int S, const double* background,
+                                double* rhs, hipStream_t s);
+hipError_t launch_charge_bound32(const DevParticles32& P, int64_t n, const double* species, PushPartial* partial,
+                                 hipStream_t s);
+hipError_t launch_ke32(const DevParticles32& P, int64_t n, const double* species, int S, int want_sum,
+                       PushPartial* partial, hipStream_t s);
+hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst, const int32_t* perm, int64_t n,
+                            hipStream_t s);
+// FP64 <-> precision-form copies of particles [off, off+n) (to32: P64 -> P32).
+hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n, bool to32,
+                          hipStream_t s);
+hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s);
+// Ownership check: adds to *bad the live particles whose element does not
+// contain their position under the FP64 barycentric test.
+hipError_t launch_check_owner(const DevMesh& m, const DevParticles& P, int64_t n, unsigned long long* bad,
+                              hipStream_t s);
+hipError_t launch_check_owner32(const DevMesh& m, const DevParticles32& P, int64_t n, unsigned long long* bad,
+                                hipStream_t s);
+
+// Mixed-precision CG: FP32 CG inner solves inside FP64 iterative refinement.
+struct IRState {
+  int done, converged, outer, in_done, in_iters, total_inner, max_iter, pad;
+  double bb, rr, tol2, in_tol2, in_rho0, in_rho, scale;
+};
+struct IRBuffers {
+  double *x, *b, *r;            // FP64 solution, right-hand side, residual
+  float *d, *r32, *p0, *p1, *q; // FP32 inner correction, residual, directions, A p
+  double *part_a, *part_b;      // [2*nchunks] each: b.b and r.r; inner r.r and p.q
+  double* rho_hist;             // [max_iter+1]
+  IRState* st;
+  int nchunks;
+};
+// Outer step: r = b - A x (FP64) and the convergence test; starts an inner solve
+// on r/|r| unless converged.
+hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bool first, double tol, int max_iter,
+                           double in_tol_min, hipStream_t s);
+hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s);
+// x += |r| d (FP64).
+hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, hipStream_t s);
+
 struct SortBuffers {
   int32_t* count;    // [ne+1]
   int32_t* start;    // [ne+1]
@@ -185,8 +286,10 @@ struct SortBuffers {
 };
 // Counting sort by element, as a permutation: count (unless counts_ready), scan
 // into sb.start, then perm[start[e] + rank] = position of each live particle.
-hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool counts_ready, SortBuffers& sb,
-                       int32_t* perm, hipStream_t s);
+// by_bucket: the key is m.bucket[elem] instead of elem (counts and starts are
+// then per bucket).
+hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool by_bucket, bool counts_ready,
+                       SortBuffers& sb, int32_t* perm, hipStream_t s);
 // dst[i] = src[perm[i]] for every field, i < n.
 hipError_t launch_permute(const DevParticles& src, const DevParticles& dst, const int32_t* perm, int64_t n,
                           hipStream_t s);

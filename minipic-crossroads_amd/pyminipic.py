@@ -24,7 +24,7 @@ EXPORTS = [
     "mpic_particles_upload", "mpic_particles_download", "mpic_num_particles", "mpic_fill", "mpic_weight_charge",
     "mpic_solve", "mpic_weight_efield", "mpic_accel", "mpic_move", "mpic_compute_ke", "mpic_sort_by_element",
     "mpic_time_step", "mpic_get_rhs", "mpic_get_phi", "mpic_set_phi", "mpic_get_efield", "mpic_charge_scale",
-    "mpic_kernel_time",
+    "mpic_check_ownership", "mpic_kernel_time",
 ]
 
 
@@ -39,7 +39,7 @@ class MeshDesc(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("form", C.c_int), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("kill_on_wall", C.c_int),
-                ("sort_period", C.c_int), ("num_species", C.c_int),
+                ("sort_period", C.c_int), ("sort_bucket", C.c_int), ("num_species", C.c_int),
                 ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES)]
 
 
@@ -50,7 +50,7 @@ class SoaHost(C.Structure):
 
 class MoveStats(C.Structure):
     _fields_ = [("crossings", C.c_int64), ("killed", C.c_int64), ("capped", C.c_int64),
-                ("nonpositive", C.c_int64), ("reflections", C.c_int64)]
+                ("nonpositive", C.c_int64), ("reflections", C.c_int64), ("relocated", C.c_int64)]
 
 
 class StepStats(C.Structure):
@@ -103,6 +103,7 @@ def lib():
             "mpic_set_phi": (i32, [P, P]),
             "mpic_get_efield": (i32, [P, P, i64]),
             "mpic_charge_scale": (i32, [P, P]),
+            "mpic_check_ownership": (i32, [P, P]),
             "mpic_kernel_time": (i32, [P, i32, P]),
         }
         for name, (res, args) in sig.items():
@@ -299,6 +300,14 @@ class Context:
         s = C.c_int()
         self._chk(lib().mpic_charge_scale(self.h, C.byref(s)))
         return s.value
+
+    def set_form(self, form):
+        self._chk(lib().mpic_set_form(self.h, form))
+
+    def check_ownership(self):
+        n = C.c_int64()
+        self._chk(lib().mpic_check_ownership(self.h, C.byref(n)))
+        return n.value
 
     def kernel_time(self, which):
         ms = C.c_double()
