@@ -504,20 +504,14 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
   if (!c || cap <= 0 || cap > ((int64_t)1 << 31) - 1) return fail(c, MPIC_ERR_ARG, "mpic_particles_reserve: bad capacity");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   free_particles(c);
-  const size_t per = 10 * sizeof(double) + sizeof(int32_t) + sizeof(uint32_t) + 1;
   for (int k = 0; k < 2; ++k) {
     char* base = nullptr;
-    HIPCHK(c, hipMalloc((void**)&base, per * (size_t)cap + 1024));
+    HIPCHK(c, hipMalloc((void**)&base, (sizeof(Rec64) + sizeof(int32_t)) * (size_t)cap + 1024));
     c->pmem[k] = base;
     DevParticles& P = c->P[k];
     P.cap = cap;
-    P.x = (double*)base;
-    P.xr = P.x + 3 * cap;
-    P.v = P.xr + 3 * cap;
-    P.w = P.v + 3 * cap;
-    P.elem = (int32_t*)(P.w + cap);
-    P.id = (uint32_t*)(P.elem + cap);
-    P.type = (uint8_t*)(P.id + cap);
+    P.r = (Rec64*)base;
+    P.elem = (int32_t*)(P.r + cap);
   }
   HIPCHK(c, hipMalloc((void**)&c->perm, sizeof(int32_t) * (size_t)cap));
   c->cap = cap;
@@ -541,25 +535,30 @@ int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
   if (!c || !h || n < 0 || !h->x || !h->x_ref || !h->v || !h->elem || !h->weight || !h->type || h->stride < n)
     return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: bad arguments");
   if (n > c->cap) return fail(c, MPIC_ERR_CAPACITY, "mpic_particles_upload: capacity exceeded");
-  std::vector<uint8_t> t(n);
-  std::vector<uint32_t> ids(n);
+  // Records are packed on the host and copied in one transfer.
+  std::vector<Rec64> rec(n);
   for (int64_t p = 0; p < n; ++p) {
     if (h->type[p] < 0 || h->type[p] >= c->cfg.num_species)
       return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: species out of range");
     if (h->elem[p] < -1 || h->elem[p] >= c->m.ne) return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: element out of range");
-    t[p] = (uint8_t)h->type[p];
-    ids[p] = h->id ? h->id[p] : (uint32_t)p;
+    Rec64& r = rec[p];
+    for (int d = 0; d < 3; ++d) {
+      r.x[d] = h->x[d * h->stride + p];
+      r.xr[d] = h->x_ref[d * h->stride + p];
+      r.v[d] = h->v[d * h->stride + p];
+    }
+    r.w = h->weight[p];
+    r.elem = h->elem[p];
+    r.id = h->id ? h->id[p] : (uint32_t)p;
+    r.type = (uint32_t)h->type[p];
+    r.pad = 0;
   }
   DevParticles& P = c->P[c->cur];
-  for (int d = 0; d < 3; ++d) {
-    HIPCHK(c, hipMemcpy(P.x + d * P.cap, h->x + d * h->stride, n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(P.xr + d * P.cap, h->x_ref + d * h->stride, n * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(P.v + d * P.cap, h->v + d * h->stride, n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n) {
+    HIPCHK(c, hipMemcpy(P.r, rec.data(), n * sizeof(Rec64), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(P.elem, h->elem, n * 4, hipMemcpyHostToDevice));
   }
-  HIPCHK(c, hipMemcpy(P.w, h->weight, n * 8, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(P.elem, h->elem, n * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(P.type, t.data(), n, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(P.id, ids.data(), n * 4, hipMemcpyHostToDevice));
   c->n = n;
   c->ke_max_prev = 0;
   int rc;
@@ -576,18 +575,19 @@ int mpic_particles_download(mpic_ctx* c, mpic_soa_host* h, int64_t* n) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevParticles& P = c->P[c->cur];
   const int64_t cnt = c->n;
-  for (int d = 0; d < 3; ++d) {
-    if (h->x) HIPCHK(c, hipMemcpy(h->x + d * h->stride, P.x + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
-    if (h->x_ref) HIPCHK(c, hipMemcpy(h->x_ref + d * h->stride, P.xr + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
-    if (h->v) HIPCHK(c, hipMemcpy(h->v + d * h->stride, P.v + d * P.cap, cnt * 8, hipMemcpyDeviceToHost));
-  }
-  if (h->weight) HIPCHK(c, hipMemcpy(h->weight, P.w, cnt * 8, hipMemcpyDeviceToHost));
-  if (h->elem) HIPCHK(c, hipMemcpy(h->elem, P.elem, cnt * 4, hipMemcpyDeviceToHost));
-  if (h->id) HIPCHK(c, hipMemcpy(h->id, P.id, cnt * 4, hipMemcpyDeviceToHost));
-  if (h->type) {
-    std::vector<uint8_t> t(cnt);
-    HIPCHK(c, hipMemcpy(t.data(), P.type, cnt, hipMemcpyDeviceToHost));
-    for (int64_t p = 0; p < cnt; ++p) h->type[p] = t[p];
+  std::vector<Rec64> rec(cnt);
+  if (cnt) HIPCHK(c, hipMemcpy(rec.data(), P.r, cnt * sizeof(Rec64), hipMemcpyDeviceToHost));
+  for (int64_t p = 0; p < cnt; ++p) {
+    const Rec64& r = rec[p];
+    for (int d = 0; d < 3; ++d) {
+      if (h->x) h->x[d * h->stride + p] = r.x[d];
+      if (h->x_ref) h->x_ref[d * h->stride + p] = r.xr[d];
+      if (h->v) h->v[d * h->stride + p] = r.v[d];
+    }
+    if (h->weight) h->weight[p] = r.w;
+    if (h->elem) h->elem[p] = r.elem;
+    if (h->id) h->id[p] = r.id;
+    if (h->type) h->type[p] = (int32_t)r.type;
   }
   if (n) *n = cnt;
   return MPIC_OK;

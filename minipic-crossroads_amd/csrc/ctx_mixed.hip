@@ -20,20 +20,14 @@ constexpr double IR_IN_TOL_MIN = 1e-5;
 int alloc_particles32(mpic_ctx* c) {
   if (c->cap == 0 || c->pmem32[0]) return MPIC_OK;
   const int64_t cap = c->cap;
-  const size_t per = sizeof(double) + 9 * sizeof(float) + sizeof(int32_t) + sizeof(uint32_t) + 1;
   for (int k = 0; k < 2; ++k) {
     char* base = nullptr;
-    HIPCHK(c, hipMalloc((void**)&base, per * (size_t)cap + 1024));
+    HIPCHK(c, hipMalloc((void**)&base, (sizeof(Rec32) + sizeof(int32_t)) * (size_t)cap + 1024));
     c->pmem32[k] = base;
     DevParticles32& P = c->P32[k];
     P.cap = cap;
-    P.w = (double*)base;
-    P.x = (float*)(P.w + cap);
-    P.xr = P.x + 3 * cap;
-    P.v = P.xr + 3 * cap;
-    P.elem = (int32_t*)(P.v + 3 * cap);
-    P.id = (uint32_t*)(P.elem + cap);
-    P.type = (uint8_t*)(P.id + cap);
+    P.r = (Rec32*)base;
+    P.elem = (int32_t*)(P.r + cap);
   }
   c->cur32 = 0;
   return MPIC_OK;

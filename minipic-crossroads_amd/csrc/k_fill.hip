@@ -75,16 +75,19 @@ __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
   double wp = a.density * vol / a.nppe;
   if (a.pert_amp != 0.0) wp = wp * (1.0 + a.pert_amp * sin(a.pert_k * (ph[0] - a.m.lo[0])));
   const int64_t p = a.offset + q;
-  const int64_t cap = a.P.cap;
+  Rec64 o;
   for (int d = 0; d < 3; ++d) {
-    a.P.x[d * cap + p] = ph[d];
-    a.P.xr[d * cap + p] = ref[d];
-    a.P.v[d * cap + p] = vv[d];
+    o.x[d] = ph[d];
+    o.xr[d] = ref[d];
+    o.v[d] = vv[d];
   }
+  o.w = wp;
+  o.elem = e;
+  o.id = (uint32_t)p;
+  o.type = (uint32_t)a.type;
+  o.pad = 0;
+  store_rec(a.P.r, p, o);
   a.P.elem[p] = e;
-  a.P.w[p] = wp;
-  a.P.type[p] = (uint8_t)a.type;
-  a.P.id[p] = (uint32_t)p;
 }
 
 }  // namespace

@@ -241,8 +241,6 @@ __global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __re
 __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
   __shared__ DepTable64 table;
   dep64_clear(table);
-  const int64_t cap = a.P.cap;
-  const int64_t qcap = a.Q.cap;
   const bool gather = a.perm != nullptr;
   WalkCount wc{};
   long long killed = 0;
@@ -260,17 +258,18 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
     int t = 0;
     if (p < hi) {
       const int64_t src = gather ? (int64_t)a.perm[p] : p;
-      int e = a.P.elem[src];
+      Rec32 rc = load_rec(a.P.r, src);
+      int e = rc.elem;
       if (e >= 0) {
         float X[3], R[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          X[d] = a.P.x[d * cap + src];
-          R[d] = a.P.xr[d * cap + src];
-          V[d] = a.P.v[d * cap + src];
+          X[d] = rc.x[d];
+          R[d] = rc.xr[d];
+          V[d] = rc.v[d];
         }
-        t = a.P.type[src];
-        w = a.P.w[src];
+        t = (int)rc.type;
+        w = rc.w;
         const float4 E = a.m32.E[e];
         float g[12];
         load12f(a.m32.geo, e, g);
@@ -288,19 +287,17 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
         if (alive && near_face(R)) locate64(a.m, e, X, R, wc.relocated);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          a.Q.x[d * qcap + p] = X[d];
-          a.Q.xr[d * qcap + p] = R[d];
-          a.Q.v[d * qcap + p] = V[d];
+          rc.x[d] = X[d];
+          rc.xr[d] = R[d];
+          rc.v[d] = V[d];
         }
         enew = alive ? e : -1;
         killed += alive ? 0 : 1;
+        rc.elem = enew;
+        store_rec(a.Q.r, p, rc);
         a.Q.elem[p] = enew;
-        if (gather) {
-          a.Q.w[p] = w;
-          a.Q.type[p] = (uint8_t)t;
-          a.Q.id[p] = a.P.id[src];
-        }
       } else if (gather) {
+        store_rec(a.Q.r, p, rc);
         a.Q.elem[p] = -1;
       }
     }
@@ -330,8 +327,8 @@ __global__ __launch_bounds__(BLOCK) void k_deposit32(DevMesh m, DevParticles32 P
   for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
     const int64_t p = base + threadIdx.x;
     if (p < hi) {
-      const int e = P.elem[p];
-      if (e >= 0) dep64_add(table, acc, nullptr, nullptr, e, to_fixed64(P.w[p] * sp[P.type[p]], S));
+      const Rec32 rc = load_rec(P.r, p);
+      if (rc.elem >= 0) dep64_add(table, acc, nullptr, nullptr, rc.elem, to_fixed64(rc.w * sp[rc.type], S));
     }
     if ((chunk + 1) % FLUSH_CHUNKS == 0) dep64_flush(table, acc, nullptr, nullptr);
   }
@@ -357,8 +354,9 @@ __global__ __launch_bounds__(BLOCK) void k_charge_bound32(DevParticles32 P, int6
   unsigned long long qmax = 0;
   WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
-    if (P.elem[p] < 0) continue;
-    const unsigned long long qb = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
+    const Rec32 rc = load_rec(P.r, p);
+    if (rc.elem < 0) continue;
+    const unsigned long long qb = __double_as_longlong(fabs(rc.w * sp[rc.type]));
     qmax = qb > qmax ? qb : qmax;
     live.crossings++;
   }
@@ -371,10 +369,10 @@ __global__ __launch_bounds__(BLOCK) void k_ke32(DevParticles32 P, int64_t n, con
   unsigned long long kmax = 0;
   WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
-    if (P.elem[p] < 0) continue;
-    double V[3];
-    for (int d = 0; d < 3; ++d) V[d] = (double)P.v[d * P.cap + p];
-    const double term = ke_term(P.w[p], sp[16 + P.type[p]], V);
+    const Rec32 rc = load_rec(P.r, p);
+    if (rc.elem < 0) continue;
+    const double V[3] = {(double)rc.v[0], (double)rc.v[1], (double)rc.v[2]};
+    const double term = ke_term(rc.w, sp[16 + rc.type], V);
     if (want_sum) ke = fx_add(ke, to_fixed(term, S));
     const unsigned long long tb = __double_as_longlong(term);
     kmax = tb > kmax ? tb : kmax;
@@ -387,17 +385,9 @@ __global__ __launch_bounds__(BLOCK) void k_permute32(DevParticles32 src, DevPart
                                                      const int32_t* __restrict__ perm, int64_t n) {
   const int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (q >= n) return;
-  const int64_t p = perm[q];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    dst.x[d * dst.cap + q] = src.x[d * src.cap + p];
-    dst.xr[d * dst.cap + q] = src.xr[d * src.cap + p];
-    dst.v[d * dst.cap + q] = src.v[d * src.cap + p];
-  }
-  dst.w[q] = src.w[p];
-  dst.elem[q] = src.elem[p];
-  dst.type[q] = src.type[p];
-  dst.id[q] = src.id[p];
+  const Rec32 r = load_rec(src.r, perm[q]);
+  store_rec(dst.r, q, r);
+  dst.elem[q] = r.elem;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticles32 P32, int64_t off, int64_t n,
@@ -406,27 +396,37 @@ __global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticle
   if (j >= n) return;
   const int64_t i = off + j;
   if (to32) {
+    const Rec64 a = load_rec(P64.r, i);
+    Rec32 b;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      P32.x[d * P32.cap + i] = (float)P64.x[d * P64.cap + i];
-      P32.xr[d * P32.cap + i] = (float)P64.xr[d * P64.cap + i];
-      P32.v[d * P32.cap + i] = (float)P64.v[d * P64.cap + i];
+      b.x[d] = (float)a.x[d];
+      b.xr[d] = (float)a.xr[d];
+      b.v[d] = (float)a.v[d];
     }
-    P32.w[i] = P64.w[i];
-    P32.elem[i] = P64.elem[i];
-    P32.type[i] = P64.type[i];
-    P32.id[i] = P64.id[i];
+    b.w = a.w;
+    b.elem = a.elem;
+    b.id = a.id;
+    b.type = a.type;
+    b.pad = 0;
+    store_rec(P32.r, i, b);
+    P32.elem[i] = a.elem;
   } else {
+    const Rec32 b = load_rec(P32.r, i);
+    Rec64 a;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      P64.x[d * P64.cap + i] = (double)P32.x[d * P32.cap + i];
-      P64.xr[d * P64.cap + i] = (double)P32.xr[d * P32.cap + i];
-      P64.v[d * P64.cap + i] = (double)P32.v[d * P32.cap + i];
+      a.x[d] = (double)b.x[d];
+      a.xr[d] = (double)b.xr[d];
+      a.v[d] = (double)b.v[d];
     }
-    P64.w[i] = P32.w[i];
-    P64.elem[i] = P32.elem[i];
-    P64.type[i] = P32.type[i];
-    P64.id[i] = P32.id[i];
+    a.w = b.w;
+    a.elem = b.elem;
+    a.id = b.id;
+    a.type = b.type;
+    a.pad = 0;
+    store_rec(P64.r, i, a);
+    P64.elem[i] = b.elem;
   }
 }
 
@@ -443,11 +443,12 @@ __global__ __launch_bounds__(BLOCK) void k_check_owner(DevMesh m, PT P, int64_t 
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   int mismatch = 0;
   if (p < n) {
-    const int e = P.elem[p];
+    const auto rc = load_rec(P.r, p);
+    const int e = rc.elem;
     if (e >= 0) {
       double g[12];
       load12(m.geo, e, g);
-      const double dl[3] = {(double)P.x[p] - g[9], (double)P.x[P.cap + p] - g[10], (double)P.x[2 * P.cap + p] - g[11]};
+      const double dl[3] = {(double)rc.x[0] - g[9], (double)rc.x[1] - g[10], (double)rc.x[2] - g[11]};
       double r[3];
       p2r_vec(g, dl, r);
       const double lam[4] = {1.0 - r[0] - r[1] - r[2], r[0], r[1], r[2]};
@@ -470,7 +471,8 @@ hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStr
 }
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  if (!a.m32.geo || !a.m32.E || !a.P.x || !a.Q.x || !a.acc || !a.partial || !a.species || !a.m.bucket)
+  if (!a.m32.geo || !a.m32.E || !a.P.r || !a.Q.r || !a.P.elem || !a.Q.elem || !a.acc || !a.partial || !a.species ||
+      !a.m.bucket)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_push32, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, a);
   return hipGetLastError();

@@ -185,8 +185,6 @@ struct KArgs {
 __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   __shared__ DepTable table;
   dep_table_clear(table);
-  const int64_t cap = a.P.cap;
-  const int64_t qcap = a.Q.cap;
   const bool gather = a.perm != nullptr;
   WalkCount wc{};
   long long killed = 0;
@@ -203,17 +201,18 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     int t = 0;
     if (p < hi) {
       const int64_t src = gather ? (int64_t)a.perm[p] : p;  // sorted order: gather through the permutation
-      int e = a.P.elem[src];
+      Rec64 rc = load_rec(a.P.r, src);
+      int e = rc.elem;
       if (e >= 0) {
         double X[3], R[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          X[d] = a.P.x[d * cap + src];
-          R[d] = a.P.xr[d * cap + src];
-          V[d] = a.P.v[d * cap + src];
+          X[d] = rc.x[d];
+          R[d] = rc.xr[d];
+          V[d] = rc.v[d];
         }
-        t = a.P.type[src];
-        w = a.P.w[src];
+        t = (int)rc.type;
+        w = rc.w;
         double E[3];
         gather_E(a.m, a.phi, e, R, E);
         double g[12];
@@ -223,19 +222,17 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          a.Q.x[d * qcap + p] = X[d];
-          a.Q.xr[d * qcap + p] = R[d];
-          a.Q.v[d * qcap + p] = V[d];
+          rc.x[d] = X[d];
+          rc.xr[d] = R[d];
+          rc.v[d] = V[d];
         }
         enew = alive ? e : -1;
         killed += alive ? 0 : 1;
+        rc.elem = enew;
+        store_rec(a.Q.r, p, rc);
         a.Q.elem[p] = enew;
-        if (gather) {
-          a.Q.w[p] = w;
-          a.Q.type[p] = (uint8_t)t;
-          a.Q.id[p] = a.P.id[src];
-        }
       } else if (gather) {
+        store_rec(a.Q.r, p, rc);
         a.Q.elem[p] = -1;
       }
     }
@@ -258,12 +255,12 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
 __global__ __launch_bounds__(BLOCK) void k_gather(KArgs a) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (p >= a.n) return;
-  const int e = a.P.elem[p];
+  const Rec64 rc = load_rec(a.P.r, p);
+  const int e = rc.elem;
   if (e < 0) return;
   const int64_t cap = a.P.cap;
-  double R[3], E[3];
-  for (int d = 0; d < 3; ++d) R[d] = a.P.xr[d * cap + p];
-  gather_E(a.m, a.phi, e, R, E);
+  double E[3];
+  gather_E(a.m, a.phi, e, rc.xr, E);
   for (int d = 0; d < 3; ++d) a.E[d * cap + p] = E[d];
 }
 
@@ -271,21 +268,23 @@ __global__ __launch_bounds__(BLOCK) void k_gather(KArgs a) {
 __global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (p >= a.n) return;
-  const int e = a.P.elem[p];
+  Rec64 rc = load_rec(a.P.r, p);
+  const int e = rc.elem;
   if (e < 0) return;
   const int64_t cap = a.P.cap;
   double V[3], E[3], VR[3];
   for (int d = 0; d < 3; ++d) {
-    V[d] = a.P.v[d * cap + p];
+    V[d] = rc.v[d];
     E[d] = a.E[d * cap + p];
   }
   double g[12];
   load12(a.m.geo, e, g);
-  kick(g, V, E, a.sp[32 + a.P.type[p]], a.dt, a.scale, VR);
+  kick(g, V, E, a.sp[32 + rc.type], a.dt, a.scale, VR);
   for (int d = 0; d < 3; ++d) {
-    a.P.v[d * cap + p] = V[d];
+    rc.v[d] = V[d];
     a.vr[d * cap + p] = VR[d];
   }
+  store_rec(a.P.r, p, rc);
 }
 
 // MoveParticle: the tet walk from the stored v and v_ref (no deposit).
@@ -296,22 +295,25 @@ __global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < a.n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t p = base + threadIdx.x;
     if (p >= a.n) continue;
-    int e = a.P.elem[p];
+    Rec64 rc = load_rec(a.P.r, p);
+    int e = rc.elem;
     if (e < 0) continue;
     double X[3], R[3], V[3], VR[3];
     for (int d = 0; d < 3; ++d) {
-      X[d] = a.P.x[d * cap + p];
-      R[d] = a.P.xr[d * cap + p];
-      V[d] = a.P.v[d * cap + p];
+      X[d] = rc.x[d];
+      R[d] = rc.xr[d];
+      V[d] = rc.v[d];
       VR[d] = a.vr[d * cap + p];
     }
     const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
     for (int d = 0; d < 3; ++d) {
-      a.P.x[d * cap + p] = X[d];
-      a.P.xr[d * cap + p] = R[d];
-      a.P.v[d * cap + p] = V[d];
+      rc.x[d] = X[d];
+      rc.xr[d] = R[d];
+      rc.v[d] = V[d];
     }
-    a.P.elem[p] = alive ? e : -1;
+    rc.elem = alive ? e : -1;
+    store_rec(a.P.r, p, rc);
+    a.P.elem[p] = rc.elem;
     killed += alive ? 0 : 1;
   }
   block_partial(a.partial, fx128{0, 0}, 0ull, wc, killed);
@@ -329,10 +331,11 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, in
   for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
     const int64_t p = base + threadIdx.x;
     if (p < hi) {
-      const int e = P.elem[p];
+      const Rec64 rc = load_rec(P.r, p);
+      const int e = rc.elem;
       if (e >= 0) {
         fx128 c[4];
-        particle_charge(m, e, P.w[p], sp[P.type[p]], S, c);
+        particle_charge(m, e, rc.w, sp[rc.type], S, c);
         dep_table_add(table, acc, nullptr, nullptr, e, c);
       }
     }
@@ -349,10 +352,9 @@ __global__ __launch_bounds__(BLOCK) void k_ke(DevParticles P, int64_t n, const d
   unsigned long long kmax = 0;
   WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
-    if (P.elem[p] < 0) continue;
-    double V[3];
-    for (int d = 0; d < 3; ++d) V[d] = P.v[d * P.cap + p];
-    const double term = ke_term(P.w[p], sp[16 + P.type[p]], V);
+    const Rec64 rc = load_rec(P.r, p);
+    if (rc.elem < 0) continue;
+    const double term = ke_term(rc.w, sp[16 + rc.type], rc.v);
     if (want_sum) ke = fx_add(ke, to_fixed(term, S));
     const unsigned long long tb = __double_as_longlong(term);
     kmax = tb > kmax ? tb : kmax;
@@ -368,8 +370,9 @@ __global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t 
   unsigned long long qmax = 0;
   WalkCount live{};
   for (int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * BLOCK) {
-    if (P.elem[p] < 0) continue;
-    const unsigned long long qb = __double_as_longlong(fabs(P.w[p] * sp[P.type[p]]));
+    const Rec64 rc = load_rec(P.r, p);
+    if (rc.elem < 0) continue;
+    const unsigned long long qb = __double_as_longlong(fabs(rc.w * sp[rc.type]));
     qmax = qb > qmax ? qb : qmax;
     live.crossings++;
   }

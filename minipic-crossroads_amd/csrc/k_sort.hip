@@ -138,17 +138,9 @@ __global__ __launch_bounds__(BLOCK) void k_permute(DevParticles src, DevParticle
                                                    int64_t n) {
   const int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (q >= n) return;
-  const int64_t p = perm[q];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    dst.x[d * dst.cap + q] = src.x[d * src.cap + p];
-    dst.xr[d * dst.cap + q] = src.xr[d * src.cap + p];
-    dst.v[d * dst.cap + q] = src.v[d * src.cap + p];
-  }
-  dst.w[q] = src.w[p];
-  dst.elem[q] = src.elem[p];
-  dst.type[q] = src.type[p];
-  dst.id[q] = src.id[p];
+  const Rec64 r = load_rec(src.r, perm[q]);
+  store_rec(dst.r, q, r);
+  dst.elem[q] = r.elem;
 }
 
 }  // namespace

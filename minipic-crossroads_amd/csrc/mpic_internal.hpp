@@ -54,28 +54,47 @@ struct DevMesh {
   double lo[3], hi[3];
 };
 
-// FP64 particle SoA on the device.
+// Particles on the device are records (one 16-B aligned record per particle,
+// read and written as 16-B words) plus a mirrored array of owning elements for
+// the sort.  A push that reads its particles through a permutation then touches
+// one or two cache lines per particle instead of one line per field.
+
+// FP64 particle record, 96 B.
+struct alignas(16) Rec64 {
+  double x[3];    // ParticleList::x      particle_list.hpp:71
+  double xr[3];   // ParticleList::x_ref  particle_list.hpp:73
+  double v[3];    // ParticleList::v      particle_list.hpp:75
+  double w;       // ParticleList::weight particle_list.hpp:85
+  int32_t elem;   // ParticleList::ielement, -1 dead
+  uint32_t id;    // identity key (fill order)
+  uint32_t type;  // ParticleList::type
+  uint32_t pad;
+};
+static_assert(sizeof(Rec64) == 96, "FP64 particle record is six 16-B words");
+
 struct DevParticles {
-  double* x;      // [3][cap]
-  double* xr;     // [3][cap]
-  double* v;      // [3][cap]
-  double* w;      // [cap]
-  int32_t* elem;  // [cap], -1 dead
-  uint8_t* type;  // [cap]
-  uint32_t* id;   // [cap]
+  Rec64* r;       // [cap]
+  int32_t* elem;  // [cap] copy of r[].elem, read by the sort
   int64_t cap;
 };
 
-// Precision-form particle SoA: FP32 position, reference position and velocity;
-// the weight stays FP64 because charges are summed exactly from it.
+// Precision-form particle record, 64 B: FP32 position, reference position and
+// velocity; the weight stays FP64 because charges are summed exactly from it.
+struct alignas(16) Rec32 {
+  float x[3];
+  int32_t elem;
+  float xr[3];
+  uint32_t id;
+  float v[3];
+  uint32_t type;
+  double w;
+  double pad;
+};
+static_assert(sizeof(Rec32) == 64, "precision-form particle record is four 16-B words");
+
 struct DevParticles32 {
-  float* x;       // [3][cap]
-  float* xr;      // [3][cap]
-  float* v;       // [3][cap]
-  double* w;      // [cap]
-  int32_t* elem;  // [cap], -1 dead
-  uint8_t* type;  // [cap]
-  uint32_t* id;   // [cap]
+  Rec32* r;       // [cap]
+  int32_t* elem;  // [cap] copy of r[].elem, read by the sort
   int64_t cap;
 };
 
