@@ -10,6 +10,11 @@ dt = 0.05.  Synthetic inputs generated on the device (seeded).
 One step = PIC::time_step: charge deposit, CG solve, E gather, Boris kick,
 tet-walk relocation, KE, amortised counting sort.  The metric is
 particle-steps per second over exactly --steps timed steps after --warmup.
+Both forms run the same workload from the same fill: the precision form
+(FP32 push/gather/walk, FP64 point location near faces, 64-bit fixed-point
+deposit, FP32 CG inside FP64 refinement) is the headline value, the FP64 form
+(bit-identical to the reference's arithmetic) is reported beside it with the
+ratio, and the precision form's errors against the FP64 form are reported.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -27,15 +32,16 @@ ORACLE = os.path.join(ROOT, "oracle")
 
 HBM_PEAK_GBS = 8000.0
 DENSITY = 2.0 / 3.0 * math.pi * math.pi  # main.cpp:68
-# Algorithmic bytes per particle of the fused FP64 push: reads x, x_ref, v
-# (3 x 24 B), elem (4 B), weight (8 B), species (1 B); writes x, x_ref, v, elem.
-PUSH_BYTES_FP64 = 3 * 24 + 4 + 8 + 1 + 3 * 24 + 4
+# Algorithmic bytes per particle of the fused push: reads x, x_ref, v, elem,
+# weight, species; writes x, x_ref, v, elem.
+PUSH_BYTES_FP64 = 3 * 24 + 4 + 8 + 1 + 3 * 24 + 4        # 161 B
+PUSH_BYTES_MIXED = 3 * 12 + 4 + 8 + 1 + 3 * 12 + 4        # 89 B (FP32 state, FP64 weight)
 
 
-def measured_traffic():
+def measured_traffic(kernel):
     """HBM bytes per push launch measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE
     runs recorded in profiles/ (None when no such summary exists)."""
-    path = os.path.join(ROOT, "profiles", "k_push_fused_fp64_c2_traffic.md")
+    path = os.path.join(ROOT, "profiles", "%s_c2_traffic.md" % kernel)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -65,21 +71,30 @@ def gpu_sync():
         pass
 
 
-def run_gpu(args):
+def run_gpu(args, form):
+    """Fill, one solve on the fill (same particles in both forms), then warmup
+    and exactly args.steps timed time steps."""
+    import numpy as np
     mp = load("pyminipic", os.path.join(PKG, "pyminipic.py"))
+    name = "mixed" if form == mp.FORM_MIXED else "fp64"
     n = args.nx
     t0 = time.perf_counter()
     mesh = mp.Mesh(n, n, n, 6, background=DENSITY)
-    log("mesh %d^3 x6: %d tets, %d nodes, nnz %d (%.1f s)" % (n, mesh.ne, mesh.nn, mesh.nnz, time.perf_counter() - t0))
-    cfg = mp.default_config(sort_period=args.sort_period, cg_tol=1e-10)
+    log("[%s] mesh %d^3 x6: %d tets, %d nodes, nnz %d (%.1f s)" % (name, n, mesh.ne, mesh.nn, mesh.nnz,
+                                                                  time.perf_counter() - t0))
+    cfg = mp.default_config(sort_period=args.sort_period, cg_tol=1e-10, form=form)
     ctx = mp.Context(mesh.desc, cfg)
     ctx.reserve(mesh.ne * args.nppe)
     ctx.fill(args.nppe, 0, DENSITY, temp=2 * 0.1 ** 2, seed_offset=1, pert_amp=0.05)
     npart = ctx.num_particles()
-    log("filled %d particles" % npart)
+    ctx.weight_charge()
+    ctx.solve(1e-10, 1000)
+    phi_fill = ctx.phi()
+    log("[%s] filled %d particles" % (name, npart))
     for i in range(args.warmup):
         st = ctx.time_step(args.dt)
-        log("warmup %d: cg %d iters, push %.2f ms, solve %.2f ms" % (i, st["cg_iters"], st["ms_push"], st["ms_solve"]))
+        log("[%s] warmup %d: cg %d iters, push %.2f ms, solve %.2f ms" % (name, i, st["cg_iters"], st["ms_push"],
+                                                                       st["ms_solve"]))
     gpu_sync()
     stats = []
     push_ms = []
@@ -91,10 +106,24 @@ def run_gpu(args):
     gpu_sync()
     wall = time.perf_counter() - t0
     for i, st in enumerate(stats):
-        log("step %d: cg %d iters %.2f ms, deposit %.2f, push %.2f, sort %.2f, KE %.6e" % (
-            i, st["cg_iters"], st["ms_solve"], st["ms_deposit"], st["ms_push"], st["ms_sort"], st["kinetic_energy"]))
+        log("[%s] step %d: cg %d iters %.2f ms, deposit %.2f, push %.2f, sort %.2f, KE %.9e" % (
+            name, i, st["cg_iters"], st["ms_solve"], st["ms_deposit"], st["ms_push"], st["ms_sort"],
+            st["kinetic_energy"]))
+    out = dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nnz,
+               phi_fill=phi_fill, phi_end=ctx.phi(), rhs_end=ctx.rhs(), owner_mismatch=ctx.check_ownership())
     ctx.close()
-    return dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nnz)
+    return out
+
+
+def summary(g, K, bytes_per_particle):
+    push_ms = sum(g["push_ms"]) / len(g["push_ms"])
+    achieved = g["npart"] * bytes_per_particle / (push_ms * 1e-3) / 1e9
+    st = g["stats"]
+    return dict(value=g["npart"] * K / g["wall"], ms_per_step=g["wall"] / K * 1e3, push_ms=push_ms,
+                push_gbs=achieved, cg_iters_mean=sum(s["cg_iters"] for s in st) / K,
+                cg_ms_per_step=sum(s["ms_solve"] for s in st) / K,
+                sort_ms_per_step=sum(s["ms_sort"] for s in st) / K,
+                deposit_ms_per_step=sum(s["ms_deposit"] for s in st) / K)
 
 
 def run_cpu_baseline(args):
@@ -135,10 +164,11 @@ def main():
                     help="also run a 2 GiB-in/2 GiB-out elementwise kernel (torch) whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
-    g = run_gpu(args)
+    import numpy as np
+    gm = run_gpu(args, 1)
+    gd = run_gpu(args, 0)
     if args.copy_calibration:
         import torch
         a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
@@ -147,12 +177,22 @@ def main():
             torch.add(a, 1.0, out=b)  # an elementwise kernel: reads 2 GiB, writes 2 GiB
         torch.cuda.synchronize()
         log("copy calibration: 2 elementwise kernels of 2 GiB in, 2 GiB out done")
-    npart, K = g["npart"], args.steps
-    value = npart * K / g["wall"]
-    push_ms = sum(g["push_ms"]) / len(g["push_ms"])
-    achieved = npart * PUSH_BYTES_FP64 / (push_ms * 1e-3) / 1e9
-    cg_iters = [s["cg_iters"] for s in g["stats"]]
-    cg_ms = [s["ms_solve"] for s in g["stats"]]
+    K = args.steps
+    sm, sd = summary(gm, K, PUSH_BYTES_MIXED), summary(gd, K, PUSH_BYTES_FP64)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    ke_m, ke_d = gm["stats"][-1]["kinetic_energy"], gd["stats"][-1]["kinetic_energy"]
+    errors = {
+        "phi_l2_rel_same_particles": rel(gm["phi_fill"], gd["phi_fill"]),
+        "phi_l2_rel_after_steps": rel(gm["phi_end"], gd["phi_end"]),
+        # field energy 0.5 phi^T K phi = 0.5 phi . rhs at the solution
+        "field_energy_rel_after_steps": abs(float(np.dot(gm["phi_end"], gm["rhs_end"]) /
+                                                  np.dot(gd["phi_end"], gd["rhs_end"])) - 1.0),
+        "kinetic_energy_rel_after_steps": abs(ke_m - ke_d) / ke_d,
+        "owner_mismatch_fp64_location": gm["owner_mismatch"],
+        "relocated_last_step": gm["stats"][-1]["relocated"],
+        "steps_compared": args.warmup + K,
+        "reference": "FP64 form of this build (bit-identical to the FP64 oracle)",
+    }
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -160,27 +200,38 @@ def main():
         log("cpu baseline: %.3e particle-steps/s" % cpu["value"])
     out = {
         "metric": "particle-steps/sec incl. deposit+CG at 100M particles, mixed vs FP64 on 1 GPU",
-        "value": value,
+        "value": sm["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": g["wall"] / K * 1e3,
+        "ms_per_step": sm["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 (precision form: f32 push/gather/walk and inner CG, f64 refinement and point location, "
+                 "i64 deposit)",
         "data": "synthetic (seeded device fill)",
         "config": {
-            "workload": "Kuhn %d^3 x6 tets (%d), %d electrons/tet (%d particles), FP64 form, CG tol 1e-10, dt %g, "
-                        "sort every %d steps" % (args.nx, g["ne"], args.nppe, npart, args.dt, args.sort_period),
-            "form": "fp64", "tets": g["ne"], "nodes": g["nn"], "nnz": g["nnz"], "particles": npart,
-            "cg_iters_mean": sum(cg_iters) / K, "cg_ms_per_step": sum(cg_ms) / K,
-            "push_ms": push_ms,
+            "workload": "Kuhn %d^3 x6 tets (%d), %d electrons/tet (%d particles), vth 0.1, density perturbation "
+                        "0.05, CG tol 1e-10, dt %g, reorder every %d steps" % (
+                            args.nx, gm["ne"], args.nppe, gm["npart"], args.dt, args.sort_period),
+            "form": "mixed", "tets": gm["ne"], "nodes": gm["nn"], "nnz": gm["nnz"], "particles": gm["npart"],
+            "cg_iters_mean": sm["cg_iters_mean"], "cg_ms_per_step": sm["cg_ms_per_step"],
+            "push_ms": sm["push_ms"], "sort_ms_per_step": sm["sort_ms_per_step"],
         },
-        "roofline": {"bound": "hbm", "kernel": "k_push_fused", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "bytes_per_particle": PUSH_BYTES_FP64},
+        "fp64": {"value": sd["value"], "ms_per_step": sd["ms_per_step"], "cg_iters_mean": sd["cg_iters_mean"],
+                 "cg_ms_per_step": sd["cg_ms_per_step"], "push_ms": sd["push_ms"],
+                 "sort_ms_per_step": sd["sort_ms_per_step"],
+                 "push_roofline": {"kernel": "k_push_fused", "achieved": sd["push_gbs"], "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": sd["push_gbs"] / HBM_PEAK_GBS,
+                                   "bytes_per_particle": PUSH_BYTES_FP64,
+                                   "traffic": measured_traffic("k_push_fused")}},
+        "mixed_over_fp64": sm["value"] / sd["value"],
+        "errors_vs_fp64": errors,
+        "roofline": {"bound": "hbm", "kernel": "k_push32", "achieved": sm["push_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": sm["push_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic("k_push32"),
+                     "bytes_per_particle": PUSH_BYTES_MIXED},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

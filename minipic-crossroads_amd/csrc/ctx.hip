@@ -483,6 +483,7 @@ void mpic_destroy(mpic_ctx* c) {
   if (c->ir_h) (void)hipHostFree(c->ir_h);
   if (c->bad_h) (void)hipHostFree(c->bad_h);
   if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
+  if (c->ir.alpha_hist) (void)hipFree(c->ir.alpha_hist);
   if (c->start_h) (void)hipHostFree(c->start_h);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -56,11 +56,12 @@ int mixed_prepare(mpic_ctx* c) {
     int rc;
     if ((rc = dalloc(c, &geo32, (size_t)ne * 12)) || (rc = dalloc(c, &ell32, (size_t)c->ell_entries)) ||
         (rc = dalloc(c, &E32, (size_t)ne)) || (rc = dalloc(c, &c->acc64, (size_t)ne)) ||
-        (rc = dalloc(c, &c->ir.r, nn)) || (rc = dalloc(c, &c->ir.d, nn)) || (rc = dalloc(c, &c->ir.r32, nn)) ||
-        (rc = dalloc(c, &c->ir.p0, nn)) ||
-        (rc = dalloc(c, &c->ir.p1, nn)) || (rc = dalloc(c, &c->ir.q, nn)) ||
+        (rc = dalloc(c, &c->ir.r, nn)) || (rc = dalloc(c, &c->ir.d, nn)) || (rc = dalloc(c, &c->ir.p, nn)) ||
+        (rc = dalloc(c, &c->ir.rv[0], nn)) || (rc = dalloc(c, &c->ir.rv[1], nn)) ||
+        (rc = dalloc(c, &c->ir.wv[0], nn)) || (rc = dalloc(c, &c->ir.wv[1], nn)) ||
+        (rc = dalloc(c, &c->ir.sv[0], nn)) || (rc = dalloc(c, &c->ir.sv[1], nn)) ||
         (rc = dalloc(c, &c->ir.part_a, (size_t)2 * c->cg.nchunks)) ||
-        (rc = dalloc(c, &c->ir.part_b, (size_t)2 * c->cg.nchunks)) || (rc = dalloc(c, &c->ir.st, 1)) ||
+        (rc = dalloc(c, &c->ir.part_b, (size_t)4 * c->cg.nchunks)) || (rc = dalloc(c, &c->ir.st, 1)) ||
         (rc = dalloc(c, &c->bad, 1)))
       return rc;
     if (!c->background.empty() && (rc = dupload(c, &c->bg64, c->background.data(), c->background.size()))) return rc;
@@ -150,8 +151,10 @@ int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relre
   c->ir.b = c->rhs;
   if (max_iter + 1 > c->ir_hist_cap) {
     if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
-    c->ir.rho_hist = nullptr;
+    if (c->ir.alpha_hist) (void)hipFree(c->ir.alpha_hist);
+    c->ir.rho_hist = c->ir.alpha_hist = nullptr;
     HIPCHK(c, hipMalloc((void**)&c->ir.rho_hist, sizeof(double) * (size_t)(max_iter + 1)));
+    HIPCHK(c, hipMalloc((void**)&c->ir.alpha_hist, sizeof(double) * (size_t)(max_iter + 1)));
     c->ir_hist_cap = max_iter + 1;
   }
   std::vector<int> passes;

@@ -67,6 +67,53 @@ __device__ __forceinline__ double ell_row(const DevMesh& m, int row, F xval) {
   return acc;
 }
 
+// One ELL row staged in registers (columns, values and both gathered vectors),
+// loaded before anything waits on a reduction.  The sums below keep the
+// sequential CSR order of ell_row exactly; rows longer than ROW_REGS read their
+// tail directly.
+constexpr int ROW_REGS = 16;
+
+struct Row64 {
+  int len;
+  double v[ROW_REGS], a[ROW_REGS], b[ROW_REGS];
+};
+
+__device__ __forceinline__ void stage_row(const DevMesh& m, int row, const double* __restrict__ va,
+                                          const double* __restrict__ vb, Row64& R) {
+  const int slice = row >> 6, lane = row & 63;
+  R.len = m.ell_len[slice];
+  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
+  const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
+  int c[ROW_REGS];
+#pragma unroll
+  for (int k = 0; k < ROW_REGS; ++k) c[k] = k < R.len ? cp[k * 64] : row;
+#pragma unroll
+  for (int k = 0; k < ROW_REGS; ++k) {
+    R.v[k] = k < R.len ? vp[k * 64] : 0.0;
+    R.a[k] = va ? va[c[k]] : 0.0;
+    R.b[k] = vb[c[k]];
+  }
+}
+
+// acc = 0; acc += v_k * x_k in CSR order, x_k = b_k (first iteration) or beta a_k + b_k.
+__device__ __forceinline__ double row_apply(const DevMesh& m, int row, const Row64& R, bool first, double beta,
+                                            const double* __restrict__ va, const double* __restrict__ vb) {
+  double acc = 0;
+#pragma unroll
+  for (int k = 0; k < ROW_REGS; ++k)
+    if (k < R.len) acc += R.v[k] * (first ? R.b[k] : beta * R.a[k] + R.b[k]);
+  if (R.len > ROW_REGS) {
+    const int slice = row >> 6, lane = row & 63;
+    const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
+    const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
+    for (int k = ROW_REGS; k < R.len; ++k) {
+      const int cc = cp[k * 64];
+      acc += vp[k * 64] * (first ? vb[cc] : beta * va[cc] + vb[cc]);
+    }
+  }
+  return acc;
+}
+
 struct CGArgs {
   DevMesh m;
   double *x, *b, *r, *pold, *pnew, *q;
@@ -99,10 +146,21 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init(CGArgs a) {
 
 // Convergence test (CG_solve.hpp:71-72,103-107; max_iter :73,113), then
 // p = beta p_old + r fused into q = A p; partials of p.q (CG_solve.hpp:77-87).
+// The row's operands are loaded before the reduction of the previous partials.
 __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
   CGState* st = a.st;
   if (st->done) return;
   const int it = a.it;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const bool row = i < a.m.nn;
+  const double* __restrict__ po = it == 0 ? nullptr : a.pold;
+  Row64 R;
+  double poi = 0, ri = 0;
+  if (row) {
+    stage_row(a.m, i, po, a.r, R);
+    poi = po ? po[i] : 0.0;
+    ri = a.r[i];
+  }
   const double rho = reduce_partials(a.part_rr, gridDim.x);
   const double rho0 = it == 0 ? reduce_partials(a.part_bb, gridDim.x) : st->rho0;
   bool stop = false, conv = false;
@@ -123,19 +181,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
     }
   }
   if (stop) return;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
   double t = 0;
-  if (i < a.m.nn) {
-    const double* __restrict__ r = a.r;
-    const double* __restrict__ po = a.pold;
+  if (row) {
     double pi, qi;
     if (it == 0) {
-      pi = r[i];
-      qi = ell_row(a.m, i, [&](int c) { return r[c]; });
+      pi = ri;
+      qi = row_apply(a.m, i, R, true, 0.0, po, a.r);
     } else {
       const double beta = rho / a.rho_hist[it - 1];
-      pi = beta * po[i] + r[i];
-      qi = ell_row(a.m, i, [&](int c) { return beta * po[c] + r[c]; });
+      pi = beta * poi + ri;
+      qi = row_apply(a.m, i, R, false, beta, po, a.r);
     }
     a.pnew[i] = pi;
     a.q[i] = qi;
@@ -148,15 +203,23 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
 // alpha = rho / p.q; x += alpha p; r += (-alpha) q; partials of r.r (CG_solve.hpp:87-100).
 __global__ __launch_bounds__(BLOCK) void k_cg_k2(CGArgs a) {
   if (a.st->done) return;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const bool row = i < a.m.nn;
+  double xi = 0, pi = 0, ri = 0, qi = 0;
+  if (row) {
+    xi = a.x[i];
+    pi = a.pnew[i];
+    ri = a.r[i];
+    qi = a.q[i];
+  }
   double alpha = reduce_partials(a.part_pq, gridDim.x);
   alpha = a.rho_hist[a.it] / alpha;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
   double t = 0;
-  if (i < a.m.nn) {
-    a.x[i] = a.x[i] + alpha * a.pnew[i];
-    const double ri = a.r[i] + -alpha * a.q[i];
-    a.r[i] = ri;
-    t += ri * ri;
+  if (row) {
+    a.x[i] = xi + alpha * pi;
+    const double rn = ri + -alpha * qi;
+    a.r[i] = rn;
+    t += rn * rn;
   }
   const double s = block_tree(t);
   if (threadIdx.x == 0) a.part_rr[blockIdx.x] = s;

@@ -1,15 +1,21 @@
-// Precision form of ES::solve (ES.cpp:216-231): CG (CG_solve.hpp:54-114) in FP32
+// Precision form of ES::solve (ES.cpp:216-231): conjugate gradients in FP32
 // inside FP64 iterative refinement, to the reference's stopping rule
-// |r|^2 < tol^2 |b|^2 on the FP64 residual.
+// |r|^2 < tol^2 |b|^2 (CG_solve.hpp:71-72,103-107) on the FP64 residual.
 //
 // Outer pass (FP64): r = b - A x; stop when r.r < tol^2 b.b; otherwise an inner
-// FP32 CG solves A d = r/|r| from d = 0 to the relative residual
-// max(in_tol_min, 0.3 tol |b|/|r|), and x += |r| d.  The FP32 matrix holds the
-// FP64 stiffness values rounded to FP32 in the same sliced ELL layout; inner dot
-// products accumulate in FP64.  Kernels follow the FP64 solver's structure: each
-// launch reduces the previous launch's per-workgroup partials redundantly in
-// every workgroup, workgroup 0 records the scalars, and kernels after the stop
-// return at once, so the host can queue iterations in batches.
+// FP32 solve of A d = r/|r| from d = 0 to the relative residual
+// max(in_tol_min, 0.3 tol |b|/|r|), then x += |r| d.  The FP32 matrix holds the
+// FP64 stiffness values rounded to FP32 in the same sliced ELL layout.
+//
+// The inner solve is the single-reduction form of CG (Chronopoulos and Gear):
+// with w = A r and s = A p carried as vectors, both dot products of an
+// iteration (r.r and w.r) are reduced together, so one launch per iteration
+// suffices.  A launch reduces the previous launch's per-workgroup partials
+// redundantly in every workgroup, updates its own rows (p, s, d, r), and forms
+// w = A r_new for its rows by recomputing r_new at each column from the previous
+// r, w and s (r_new = r - alpha (w + beta s_prev)), so no other workgroup's new
+// values are read.  Vectors and partials alternate between two buffers by
+// iteration parity.  Dot products accumulate in FP64.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
 
@@ -52,34 +58,45 @@ __device__ __forceinline__ double row64(const DevMesh& m, int row, const double*
 
 #pragma clang fp contract(fast)
 
-// FP32 row of A p with p = beta p_old + r formed per column.
-template <class F>
-__device__ __forceinline__ float row32(const DevMesh& m, const float* __restrict__ val, int row, F pval) {
+// One ELL row staged in registers: columns, values and up to three gathered
+// vectors are loaded before anything waits on a reduction.  Rows longer than
+// ROW_REGS read their tail directly.
+constexpr int ROW_REGS = 16;
+
+struct Row32 {
+  int len;
+  int c[ROW_REGS];
+  float v[ROW_REGS], a[ROW_REGS], b[ROW_REGS], e[ROW_REGS];
+};
+
+__device__ __forceinline__ void stage_row(const DevMesh& m, const float* __restrict__ val, int row,
+                                          const float* __restrict__ va, const float* __restrict__ vb,
+                                          const float* __restrict__ ve, Row32& R) {
   const int slice = row >> 6, lane = row & 63;
-  const int len = m.ell_len[slice];
+  R.len = m.ell_len[slice];
   const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
   const float* __restrict__ vp = val + m.ell_off[slice] + lane;
-  float acc = 0;
-  int k = 0;
-  for (; k + 4 <= len; k += 4) {
-    const int c0 = cp[(k + 0) * 64], c1 = cp[(k + 1) * 64], c2 = cp[(k + 2) * 64], c3 = cp[(k + 3) * 64];
-    const float v0 = vp[(k + 0) * 64], v1 = vp[(k + 1) * 64], v2 = vp[(k + 2) * 64], v3 = vp[(k + 3) * 64];
-    acc += v0 * pval(c0);
-    acc += v1 * pval(c1);
-    acc += v2 * pval(c2);
-    acc += v3 * pval(c3);
+#pragma unroll
+  for (int k = 0; k < ROW_REGS; ++k) R.c[k] = k < R.len ? cp[k * 64] : row;
+#pragma unroll
+  for (int k = 0; k < ROW_REGS; ++k) {
+    R.v[k] = k < R.len ? vp[k * 64] : 0.0f;
+    R.a[k] = va[R.c[k]];
+    R.b[k] = vb[R.c[k]];
+    R.e[k] = ve ? ve[R.c[k]] : 0.0f;
   }
-  for (; k < len; ++k) acc += vp[k * 64] * pval(cp[k * 64]);
-  return acc;
 }
 
 struct IRArgs {
   DevMesh m;
   const float* val32;
   double *x, *b, *r;
-  float *d, *r32, *pold, *pnew, *q;
-  double *part_bb, *part_rr, *part_irr, *part_ipq;
-  double* rho_hist;
+  float *d, *p;
+  float *r_cur, *w_cur, *s_prev;  // r_i, w_i = A r_i, s_{i-1} (read)
+  float *r_nxt, *w_nxt, *s_cur;   // r_{i+1}, w_{i+1}, s_i (written)
+  double *part_bb, *part_rr;      // outer partials
+  double *part_cur, *part_nxt;    // inner partials [2*nchunks]: r.r then w.r
+  double *rho_hist, *alpha_hist;
   IRState* st;
   int it;
 };
@@ -111,7 +128,9 @@ __global__ __launch_bounds__(BLOCK) void k_ir_resid(IRArgs a) {
   }
 }
 
-// Outer convergence test; otherwise starts the inner solve on r/|r|.
+// Outer convergence test; otherwise starts the inner solve: r_0 = r/|r| (FP32),
+// w_0 = A r_0 (each column's r_0 formed from the FP64 residual), d = 0, and the
+// partials of r_0.r_0 and w_0.r_0.
 __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min) {
   IRState* st = a.st;
   if (st->done) return;
@@ -139,31 +158,85 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
   }
   if (stop) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  double t = 0;
+  double tg = 0, td = 0;
   if (i < a.m.nn) {
-    const float ri = (float)(a.r[i] / scale);
-    a.r32[i] = ri;
+    const double inv = 1.0 / scale;
+    const int slice = i >> 6, lane = i & 63;
+    const int len = a.m.ell_len[slice];
+    const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
+    const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
+    float w = 0;
+    for (int k = 0; k < len; ++k) w += vp[k * 64] * (float)(a.r[cp[k * 64]] * inv);
+    const float ri = (float)(a.r[i] * inv);
+    a.r_nxt[i] = ri;  // r_0 and w_0 live in the parity-0 buffers
+    a.w_nxt[i] = w;
     a.d[i] = 0.0f;
-    t = (double)ri * (double)ri;
+    tg = (double)ri * (double)ri;
+    td = (double)w * (double)ri;
   }
-  const double s = block_sum(t);
-  if (threadIdx.x == 0) a.part_irr[blockIdx.x] = s;
+  const double sg = block_sum(tg);
+  const double sd = block_sum(td);
+  if (threadIdx.x == 0) {
+    a.part_nxt[blockIdx.x] = sg;
+    a.part_nxt[gridDim.x + blockIdx.x] = sd;
+  }
 }
 
-// Inner iteration, first half: stopping test, p = beta p_old + r fused into
-// q = A p, partials of p.q.
-__global__ __launch_bounds__(BLOCK) void k_ir_k1(IRArgs a) {
+// One inner iteration i (Chronopoulos-Gear CG):
+//   gamma = r_i.r_i, delta = w_i.r_i (partials of the previous launch);
+//   beta = gamma / gamma_{i-1}, alpha = gamma / (delta - beta gamma / alpha_{i-1})
+//   (i = 0: beta = 0, alpha = gamma / delta);
+//   p_i = r_i + beta p_{i-1}; s_i = w_i + beta s_{i-1}; d += alpha p_i;
+//   r_{i+1} = r_i - alpha s_i; w_{i+1} = A r_{i+1}; partials of r_{i+1}.r_{i+1}
+//   and w_{i+1}.r_{i+1}.
+__global__ __launch_bounds__(BLOCK) void k_ir_cg(IRArgs a) {
+  // Every load that does not depend on another is issued first (state, scalar
+  // history, partials, the row and its gathers); the stop test comes after, so
+  // one iteration costs about two memory round trips.
   IRState* st = a.st;
-  if (st->done || st->in_done) return;
   const int it = a.it;
-  const double rho = reduce(a.part_irr, gridDim.x);
-  const double rho0 = it == 0 ? rho : st->in_rho0;
-  const bool conv = rho == 0 || (it > 0 && rho < st->in_tol2 * rho0);
-  const bool stop = conv || it >= st->max_iter;
+  const int done = st->done | st->in_done;
+  const double rho0_prev = st->in_rho0, tol2 = st->in_tol2;
+  const int max_iter = st->max_iter;
+  const double rho_prev = it > 0 ? a.rho_hist[it - 1] : 1.0;
+  const double alpha_prev = it > 0 ? a.alpha_hist[it - 1] : 1.0;
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const bool row = i < a.m.nn;
+  const float* __restrict__ sp = it == 0 ? nullptr : a.s_prev;
+  double pg = 0, pd = 0;
+  for (int k = threadIdx.x; k < (int)gridDim.x; k += BLOCK) {
+    pg += a.part_cur[k];
+    pd += a.part_cur[gridDim.x + k];
+  }
+  Row32 R;
+  float ri = 0, wi = 0, si = 0, pi = 0, di = 0;
+  if (row) {
+    stage_row(a.m, a.val32, i, a.r_cur, a.w_cur, sp, R);
+    ri = a.r_cur[i];
+    wi = a.w_cur[i];
+    if (it > 0) {
+      si = sp[i];
+      pi = a.p[i];
+    }
+    di = a.d[i];
+  }
+  if (done) return;
+  const double gamma = block_sum(pg);
+  const double delta = block_sum(pd);
+  const double rho0 = it == 0 ? gamma : rho0_prev;
+  double beta = 0, alpha = 0, denom = delta;
+  if (it > 0) {
+    beta = gamma / rho_prev;
+    denom = delta - beta * gamma / alpha_prev;
+  }
+  const bool conv = gamma == 0 || (it > 0 && gamma < tol2 * rho0);
+  const bool stop = conv || it >= max_iter || !(denom > 0);
+  if (!stop) alpha = gamma / denom;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->in_rho = rho;
+    st->in_rho = gamma;
     if (it == 0) st->in_rho0 = rho0;
-    a.rho_hist[it] = rho;
+    a.rho_hist[it] = gamma;
+    a.alpha_hist[it] = alpha;
     if (stop) {
       st->in_done = 1;
       st->in_iters = it;
@@ -171,42 +244,42 @@ __global__ __launch_bounds__(BLOCK) void k_ir_k1(IRArgs a) {
     }
   }
   if (stop) return;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  double t = 0;
-  if (i < a.m.nn) {
-    const float* __restrict__ r = a.r32;
-    const float* __restrict__ po = a.pold;
-    float pi, qi;
-    if (it == 0) {
-      pi = r[i];
-      qi = row32(a.m, a.val32, i, [&](int c) { return r[c]; });
-    } else {
-      const float beta = (float)(rho / a.rho_hist[it - 1]);
-      pi = beta * po[i] + r[i];
-      qi = row32(a.m, a.val32, i, [&](int c) { return beta * po[c] + r[c]; });
+  const float bf = (float)beta, af = (float)alpha;
+  double tg = 0, td = 0;
+  if (row) {
+    const float p = it > 0 ? fmaf(bf, pi, ri) : ri;
+    const float s = it > 0 ? fmaf(bf, si, wi) : wi;
+    const float rn = fmaf(-af, s, ri);
+    a.p[i] = p;
+    a.s_cur[i] = s;
+    a.d[i] = fmaf(af, p, di);
+    a.r_nxt[i] = rn;
+    float w = 0;
+#pragma unroll
+    for (int k = 0; k < ROW_REGS; ++k) {
+      const float sk = it > 0 ? fmaf(bf, R.e[k], R.b[k]) : R.b[k];
+      w = fmaf(R.v[k], fmaf(-af, sk, R.a[k]), w);
     }
-    a.pnew[i] = pi;
-    a.q[i] = qi;
-    t = (double)pi * (double)qi;
+    if (R.len > ROW_REGS) {
+      const int slice = i >> 6, lane = i & 63;
+      const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
+      const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
+      for (int k = ROW_REGS; k < R.len; ++k) {
+        const int c = cp[k * 64];
+        const float sk = it > 0 ? fmaf(bf, sp[c], a.w_cur[c]) : a.w_cur[c];
+        w = fmaf(vp[k * 64], fmaf(-af, sk, a.r_cur[c]), w);
+      }
+    }
+    a.w_nxt[i] = w;
+    tg = (double)rn * (double)rn;
+    td = (double)w * (double)rn;
   }
-  const double s = block_sum(t);
-  if (threadIdx.x == 0) a.part_ipq[blockIdx.x] = s;
-}
-
-// Inner iteration, second half: alpha = rho / p.q; d += alpha p; r -= alpha q.
-__global__ __launch_bounds__(BLOCK) void k_ir_k2(IRArgs a) {
-  if (a.st->done || a.st->in_done) return;
-  const float alpha = (float)(a.rho_hist[a.it] / reduce(a.part_ipq, gridDim.x));
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  double t = 0;
-  if (i < a.m.nn) {
-    a.d[i] = a.d[i] + alpha * a.pnew[i];
-    const float ri = a.r32[i] - alpha * a.q[i];
-    a.r32[i] = ri;
-    t = (double)ri * (double)ri;
+  const double sg = block_sum(tg);
+  const double sd = block_sum(td);
+  if (threadIdx.x == 0) {
+    a.part_nxt[blockIdx.x] = sg;
+    a.part_nxt[gridDim.x + blockIdx.x] = sd;
   }
-  const double s = block_sum(t);
-  if (threadIdx.x == 0) a.part_irr[blockIdx.x] = s;
 }
 
 // x += |r| d in FP64.
@@ -216,6 +289,7 @@ __global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a) {
   if (i < a.m.nn) a.x[i] = a.x[i] + a.st->scale * (double)a.d[i];
 }
 
+// Arguments of inner iteration it (k_ir_start fills the parity-0 buffers: it = -1).
 IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
   IRArgs a;
   a.m = m;
@@ -224,15 +298,20 @@ IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
   a.b = c.b;
   a.r = c.r;
   a.d = c.d;
-  a.r32 = c.r32;
-  a.pold = (it & 1) ? c.p1 : c.p0;
-  a.pnew = (it & 1) ? c.p0 : c.p1;
-  a.q = c.q;
+  a.p = c.p;
+  const int cur = it < 0 ? 1 : (it & 1), nxt = cur ^ 1;
+  a.r_cur = c.rv[cur];
+  a.w_cur = c.wv[cur];
+  a.s_prev = c.sv[nxt];
+  a.r_nxt = c.rv[nxt];
+  a.w_nxt = c.wv[nxt];
+  a.s_cur = c.sv[cur];
   a.part_bb = c.part_a;
   a.part_rr = c.part_a + c.nchunks;
-  a.part_irr = c.part_b;
-  a.part_ipq = c.part_b + c.nchunks;
+  a.part_cur = c.part_b + (size_t)2 * c.nchunks * cur;
+  a.part_nxt = c.part_b + (size_t)2 * c.nchunks * nxt;
   a.rho_hist = c.rho_hist;
+  a.alpha_hist = c.alpha_hist;
   a.st = c.st;
   a.it = it;
   return a;
@@ -242,20 +321,18 @@ IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
 
 hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bool first, double tol, int max_iter,
                            double in_tol_min, hipStream_t s) {
-  if (!c.x || !c.b || !c.r || !c.d || !c.r32 || !c.p0 || !c.p1 || !c.q || !c.part_a || !c.part_b || !c.rho_hist ||
-      !c.st || !m32.ell_val || c.nchunks <= 0)
+  if (!c.x || !c.b || !c.r || !c.d || !c.p || !c.rv[0] || !c.rv[1] || !c.wv[0] || !c.wv[1] || !c.sv[0] || !c.sv[1] ||
+      !c.part_a || !c.part_b || !c.rho_hist || !c.alpha_hist || !c.st || !m32.ell_val || c.nchunks <= 0)
     return hipErrorInvalidValue;
   if (first) hipLaunchKernelGGL(k_ir_setup, dim3(1), dim3(1), 0, s, c.st, tol * tol, max_iter);
-  const IRArgs a = make_args(m, m32, c, 0);
+  const IRArgs a = make_args(m, m32, c, -1);
   hipLaunchKernelGGL(k_ir_resid, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_ir_start, dim3(c.nchunks), dim3(BLOCK), 0, s, a, in_tol_min);
   return hipGetLastError();
 }
 
 hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s) {
-  const IRArgs a = make_args(m, m32, c, it);
-  hipLaunchKernelGGL(k_ir_k1, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_ir_k2, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_ir_cg, dim3(c.nchunks), dim3(BLOCK), 0, s, make_args(m, m32, c, it));
   return hipGetLastError();
 }
 

@@ -281,10 +281,12 @@ struct IRState {
   double bb, rr, tol2, in_tol2, in_rho0, in_rho, scale;
 };
 struct IRBuffers {
-  double *x, *b, *r;            // FP64 solution, right-hand side, residual
-  float *d, *r32, *p0, *p1, *q; // FP32 inner correction, residual, directions, A p
-  double *part_a, *part_b;      // [2*nchunks] each: b.b and r.r; inner r.r and p.q
-  double* rho_hist;             // [max_iter+1]
+  double *x, *b, *r;              // FP64 solution, right-hand side, residual
+  float *d, *p;                   // FP32 inner correction and direction
+  float *rv[2], *wv[2], *sv[2];   // FP32 residual r, w = A r, s = A p; by iteration parity
+  double* part_a;                 // [2*nchunks]: b.b, r.r of the outer residual
+  double* part_b;                 // [4*nchunks]: r.r, w.r of the inner residual, by parity
+  double *rho_hist, *alpha_hist;  // [max_iter+1]
   IRState* st;
   int nchunks;
 };
