@@ -63,12 +63,9 @@ def load(name, path):
 
 
 def gpu_sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except ImportError:
-        pass
+    """hipDeviceSynchronize (the library's calls are stream-ordered on its own stream)."""
+    import ctypes as C
+    assert C.CDLL("libamdhip64.so").hipDeviceSynchronize() == 0
 
 
 def run_gpu(args, form):
@@ -126,6 +123,22 @@ def summary(g, K, bytes_per_particle):
                 deposit_ms_per_step=sum(s["ms_deposit"] for s in st) / K)
 
 
+def copy_calibration(nbytes=1 << 31):
+    """Two device-to-device copies of nbytes (the runtime's blit kernel): the
+    FETCH_SIZE / WRITE_SIZE counters read for a copy of known size calibrate the
+    push kernels' HBM traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: 2 device-to-device copies of %d bytes done" % nbytes)
+
+
 def run_cpu_baseline(args):
     """The FP64 oracle (restatement of the reference's OpenMP path, host solve
     single-threaded as in the reference) on one step of the same workload."""
@@ -161,7 +174,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a 2 GiB-in/2 GiB-out elementwise kernel (torch) whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")
+                    help="also run two 2 GiB device-to-device copies whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path.", file=sys.stderr)
@@ -170,13 +183,7 @@ def main():
     gm = run_gpu(args, 1)
     gd = run_gpu(args, 0)
     if args.copy_calibration:
-        import torch
-        a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(2):
-            torch.add(a, 1.0, out=b)  # an elementwise kernel: reads 2 GiB, writes 2 GiB
-        torch.cuda.synchronize()
-        log("copy calibration: 2 elementwise kernels of 2 GiB in, 2 GiB out done")
+        copy_calibration()
     K = args.steps
     sm, sd = summary(gm, K, PUSH_BYTES_MIXED), summary(gd, K, PUSH_BYTES_FP64)
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))

@@ -38,6 +38,9 @@ extern "C" {
 #define MPIC_FORM_FP64 0  /* plain form: FP64 throughout, the drop-in */
 #define MPIC_FORM_MIXED 1 /* precision form: FP32 particles, FP32 inner CG in FP64 refinement */
 
+#define MPIC_SOLVER_CG 0       /* CG_solve, CG_solve.hpp:54-114: warm start from the current potential */
+#define MPIC_SOLVER_BICGSTAB 1 /* BiCGStab_solve, CG_solve.hpp:115-206: from x = 0 (PIC::time_step, PIC.cpp:251) */
+
 #define MPIC_MAX_SPECIES 16
 
 typedef struct mpic_ctx mpic_ctx;   /* device state of one problem */
@@ -74,6 +77,7 @@ typedef struct {
   int num_species;                       /* ParticleTypeList, particle_type.hpp:58-86 */
   double species_q[MPIC_MAX_SPECIES];
   double species_m[MPIC_MAX_SPECIES];
+  int solver;                            /* MPIC_SOLVER_*: the solve of mpic_time_step (mpic_solve always runs CG) */
 } mpic_config;
 
 /* Host particle SoA (planes as described above).  Borrowed for the call. */
@@ -122,6 +126,11 @@ int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const d
                     uint64_t jitter_seed, int mapping, mpic_mesh** out);
 /* Uniform background charge density f as an rhs load vector (ES::set_analytic_RHS with a constant). */
 int mpic_mesh_set_background(mpic_mesh* mesh, double f);
+/* Load vector of a source density f: out[i] = sum over elements and cubature points c of
+ * -f(x_c) wbv[e][i][c], 0 at Dirichlet nodes (ES::set_analytic_RHS, ES.cpp:235-257).
+ * out has num_nodes entries. */
+int mpic_mesh_load_vector(const mpic_mesh* mesh, double (*f)(const double* x, void* user), void* user,
+                          double* out);
 /* Borrowed views of the mesh arrays (valid until mpic_mesh_free). */
 int mpic_mesh_describe(const mpic_mesh* mesh, mpic_mesh_desc* out);
 void mpic_mesh_free(mpic_mesh* mesh);
@@ -138,6 +147,13 @@ void mpic_destroy(mpic_ctx* ctx);
  * move return MPIC_ERR_STATE; weight_charge, solve, compute_ke, sort and
  * time_step run in that form. */
 int mpic_set_form(mpic_ctx* ctx, int form);
+/* DataWarehouse::set_kill_particles_on_wall_impact, data_warehouse.hpp:76. */
+int mpic_set_kill_on_wall(mpic_ctx* ctx, int kill);
+/* ParticleTypeList (particle_type.hpp:58-86): the charge and mass of species 0..n-1
+ * (n <= MPIC_MAX_SPECIES); q/m is formed as ParticleType's constructor does (:51). */
+int mpic_set_species(mpic_ctx* ctx, int n, const double* q, const double* m);
+/* The configuration in effect (form, tolerances, species, ...). */
+int mpic_get_config(mpic_ctx* ctx, mpic_config* cfg);
 
 /* ---- particles (ParticleList, particle_list.hpp:54-320) -------------------- */
 int mpic_particles_reserve(mpic_ctx* ctx, int64_t capacity);          /* DataWarehouse::create_particle_list */
@@ -163,7 +179,10 @@ int mpic_time_step(mpic_ctx* ctx, double dt, mpic_step_stats* stats);  /* PIC::t
 /* ---- field access ----------------------------------------------------------- */
 int mpic_get_rhs(mpic_ctx* ctx, double* rhs);        /* ES::rhs(lid), ES.hpp:63 */
 int mpic_get_phi(mpic_ctx* ctx, double* phi);        /* ES::phi(lid), ES.hpp:58 */
-int mpic_set_phi(mpic_ctx* ctx, const double* phi);
+int mpic_set_phi(mpic_ctx* ctx, const double* phi);  /* ES::set_analytic_phi, ES.cpp:259-270 */
+/* Replaces the right-hand side of the next mpic_solve (ES::set_analytic_RHS, ES.cpp:235-257);
+ * the next weight_charge or time_step recomputes it from the particles. */
+int mpic_set_rhs(mpic_ctx* ctx, const double* rhs);
 int mpic_get_efield(mpic_ctx* ctx, double* E, int64_t stride); /* ParticleList::E after weight_efield, [3][stride] */
 int mpic_charge_scale(mpic_ctx* ctx, int* bits);
 /* Live particles whose element does not contain their position under the FP64

@@ -501,6 +501,39 @@ int mpic_set_form(mpic_ctx* c, int form) {
   return MPIC_OK;
 }
 
+int mpic_set_kill_on_wall(mpic_ctx* c, int kill) {
+  if (!c) return MPIC_ERR_ARG;
+  c->cfg.kill_on_wall = kill ? 1 : 0;
+  return MPIC_OK;
+}
+
+int mpic_set_species(mpic_ctx* c, int n, const double* q, const double* m) {
+  if (!c || n < 1 || n > MPIC_MAX_SPECIES || !q || !m) return fail(c, MPIC_ERR_ARG, "mpic_set_species: bad arguments");
+  for (int t = 0; t < n; ++t)
+    if (!std::isfinite(q[t]) || !std::isfinite(m[t])) return fail(c, MPIC_ERR_ARG, "mpic_set_species: non-finite value");
+  double sp[48];
+  for (int t = 0; t < MPIC_MAX_SPECIES; ++t) {
+    c->cfg.species_q[t] = t < n ? q[t] : 0.0;
+    c->cfg.species_m[t] = t < n ? m[t] : 1.0;
+    sp[t] = c->cfg.species_q[t];
+    sp[16 + t] = c->cfg.species_m[t];
+    sp[32 + t] = c->cfg.species_m[t] != 0 ? c->cfg.species_q[t] / c->cfg.species_m[t] : 0.0;  // particle_type.hpp:51
+  }
+  c->cfg.num_species = n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->species, sp, sizeof(sp), hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = invalidate_deposit(c))) return rc;
+  c->ke_max_prev = 0;
+  return update_charge_scale(c);
+}
+
+int mpic_get_config(mpic_ctx* c, mpic_config* cfg) {
+  if (!c || !cfg) return MPIC_ERR_ARG;
+  *cfg = c->cfg;
+  return MPIC_OK;
+}
+
 int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
   if (!c || cap <= 0 || cap > ((int64_t)1 << 31) - 1) return fail(c, MPIC_ERR_ARG, "mpic_particles_reserve: bad capacity");
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -843,6 +876,13 @@ int mpic_set_phi(mpic_ctx* c, const double* phi) {
   if (!c || !phi) return MPIC_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(c->phi, phi, (size_t)c->m.nn * 8, hipMemcpyHostToDevice));
+  return MPIC_OK;
+}
+
+int mpic_set_rhs(mpic_ctx* c, const double* rhs) {
+  if (!c || !rhs) return MPIC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->rhs, rhs, (size_t)c->m.nn * 8, hipMemcpyHostToDevice));
   return MPIC_OK;
 }
 

@@ -316,6 +316,37 @@ extern "C" int mpic_mesh_set_background(mpic_mesh* m, double f) {
   return MPIC_OK;
 }
 
+// ES::set_analytic_RHS (ES.cpp:235-257): f at each cubature point (refToPhysical,
+// mesh.hpp:183-190), val_i = sum_c -f(x_c) wbv[e][i][c], zero at Dirichlet rows,
+// summed into the node in element order.
+extern "C" int mpic_mesh_load_vector(const mpic_mesh* m, double (*f)(const double* x, void* user), void* user,
+                                     double* out) {
+  if (!m || !f || !out) {
+    mpic::global_error() = "mpic_mesh_load_vector: bad arguments";
+    return MPIC_ERR_ARG;
+  }
+  std::fill(out, out + m->nn, 0.0);
+  for (int64_t e = 0; e < m->ne; ++e) {
+    double fc[4];
+    for (int c = 0; c < 4; ++c) {
+      const double* p = kCub[c];
+      const double wts[4] = {1 - p[0] - p[1] - p[2], p[0], p[1], p[2]};
+      double x[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < 4; ++i)
+        for (int d = 0; d < 3; ++d) x[d] += wts[i] * m->nodes[(size_t)m->elem_nodes[e * 4 + i] * 3 + d];
+      fc[c] = f(x, user);
+    }
+    for (int i = 0; i < 4; ++i) {
+      double v = 0;
+      for (int c = 0; c < 4; ++c) v += -fc[c] * m->wbv[e * 16 + i * 4 + c];
+      const int32_t row = m->elem_nodes[e * 4 + i];
+      if (m->boundary[row]) v = 0;
+      out[row] += v;
+    }
+  }
+  return MPIC_OK;
+}
+
 extern "C" int mpic_mesh_describe(const mpic_mesh* m, mpic_mesh_desc* d) {
   if (!m || !d) return MPIC_ERR_ARG;
   d->num_nodes = m->nn;

@@ -24,7 +24,8 @@ EXPORTS = [
     "mpic_particles_upload", "mpic_particles_download", "mpic_num_particles", "mpic_fill", "mpic_weight_charge",
     "mpic_solve", "mpic_weight_efield", "mpic_accel", "mpic_move", "mpic_compute_ke", "mpic_sort_by_element",
     "mpic_time_step", "mpic_get_rhs", "mpic_get_phi", "mpic_set_phi", "mpic_get_efield", "mpic_charge_scale",
-    "mpic_check_ownership", "mpic_kernel_time",
+    "mpic_check_ownership", "mpic_kernel_time", "mpic_mesh_load_vector", "mpic_set_kill_on_wall",
+    "mpic_set_species", "mpic_get_config", "mpic_set_rhs",
 ]
 
 
@@ -40,7 +41,8 @@ class MeshDesc(C.Structure):
 class Config(C.Structure):
     _fields_ = [("form", C.c_int), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("kill_on_wall", C.c_int),
                 ("sort_period", C.c_int), ("sort_bucket", C.c_int), ("num_species", C.c_int),
-                ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES)]
+                ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES),
+                ("solver", C.c_int)]
 
 
 class SoaHost(C.Structure):
@@ -66,6 +68,7 @@ class StepStats(C.Structure):
 
 
 _lib = None
+LOAD_FN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
 
 
 def lib():
@@ -105,6 +108,11 @@ def lib():
             "mpic_charge_scale": (i32, [P, P]),
             "mpic_check_ownership": (i32, [P, P]),
             "mpic_kernel_time": (i32, [P, i32, P]),
+            "mpic_mesh_load_vector": (i32, [P, P, P, P]),
+            "mpic_set_kill_on_wall": (i32, [P, i32]),
+            "mpic_set_species": (i32, [P, i32, P, P]),
+            "mpic_get_config": (i32, [P, P]),
+            "mpic_set_rhs": (i32, [P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -166,6 +174,15 @@ class Mesh:
         cnt = int(np.prod(shape))
         buf = (ct * cnt).from_address(ptr)
         return np.ctypeslib.as_array(buf).reshape(shape).copy()
+
+    def load_vector(self, f):
+        """ES::set_analytic_RHS load vector of the source density f(x) -> float."""
+        cb = LOAD_FN(lambda x, _u: float(f((x[0], x[1], x[2]))))
+        out = np.zeros(self.nn)
+        rc = lib().mpic_mesh_load_vector(self.h, cb, None, _p(out))
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_last_error(None).decode())
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -289,6 +306,23 @@ class Context:
     def set_phi(self, phi):
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         self._chk(lib().mpic_set_phi(self.h, _p(phi)))
+
+    def set_rhs(self, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        self._chk(lib().mpic_set_rhs(self.h, _p(rhs)))
+
+    def set_species(self, q, m):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        self._chk(lib().mpic_set_species(self.h, len(q), _p(q), _p(m)))
+
+    def set_kill_on_wall(self, kill):
+        self._chk(lib().mpic_set_kill_on_wall(self.h, int(bool(kill))))
+
+    def config(self):
+        cfg = Config()
+        self._chk(lib().mpic_get_config(self.h, C.byref(cfg)))
+        return cfg
 
     def efield(self):
         n = self.num_particles()
