@@ -77,7 +77,11 @@ typedef struct {
   int num_species;                       /* ParticleTypeList, particle_type.hpp:58-86 */
   double species_q[MPIC_MAX_SPECIES];
   double species_m[MPIC_MAX_SPECIES];
-  int solver;                            /* MPIC_SOLVER_*: the solve of mpic_time_step (mpic_solve always runs CG) */
+  int solver;                            /* MPIC_SOLVER_*: the solve of mpic_solve and mpic_time_step (FP64 form;
+                                            the precision form always runs its refinement CG) */
+  int persistent_solve;                  /* 1: a solve is one cooperative launch when its grid fits on the GPU
+                                            (same results, but slower on MI355X: its grid-wide waits cost more
+                                            than a kernel boundary); 0 (default): one launch per iteration */
 } mpic_config;
 
 /* Host particle SoA (planes as described above).  Borrowed for the call. */
@@ -168,7 +172,11 @@ int mpic_fill(mpic_ctx* ctx, int nppe, int type, double density, const double* v
 
 /* ---- phases of PIC::time_step (PIC.cpp:245-267) ----------------------------- */
 int mpic_weight_charge(mpic_ctx* ctx);                                 /* PIC::weight_charge, PIC.cpp:139-160 */
-int mpic_solve(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres); /* ES::solve, ES.cpp:216-231 + CG_solve.hpp:54-114 */
+/* ES::solve, ES.cpp:216-231, with cfg.solver: CG_solve.hpp:54-114 (warm start) or
+ * BiCGStab_solve, CG_solve.hpp:115-206 (from 0; MPIC_ERR_NOT_CONVERGED also for
+ * its early exits).  In mpic_time_step a BiCGStab that does not converge does
+ * not stop the step (PIC.cpp:251 ignores it); cg_relres reports the residual. */
+int mpic_solve(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres);
 int mpic_weight_efield(mpic_ctx* ctx);                                 /* PIC::weight_Efield, PIC.cpp:162-189 */
 int mpic_accel(mpic_ctx* ctx, double dt, double scale);                /* ParticleMove AccelParticlesTag, particle_move.hpp:364-392 */
 int mpic_move(mpic_ctx* ctx, double dt, mpic_move_stats* stats);       /* MoveParticle::execute, particle_move.hpp:72-124 */
@@ -189,6 +197,57 @@ int mpic_charge_scale(mpic_ctx* ctx, int* bits);
  * barycentric test (coordinate below -1e-9 across an interior face): the
  * ownership-mismatch count of the precision form (0 by construction). */
 int mpic_check_ownership(mpic_ctx* ctx, int64_t* mismatches);
+
+/* ---- batching form: an ensemble of independent problems --------------------------
+ * Many problems of uneven size advanced together in the precision form: one
+ * launch per phase for all problems (node charge, solve, field, push, sort,
+ * per-problem statistics), the solve one workgroup per problem, the whole step
+ * capturable in a HIP graph, and the tile shapes (push workgroups, threads per
+ * problem in the solve) chosen by mpic_batch_tune.  Each problem follows the
+ * reference's PIC::time_step (PIC.cpp:245-267) on its own mesh; particles never
+ * leave their problem.  No counterpart in the reference (one problem per run). */
+typedef struct mpic_batch mpic_batch;
+
+typedef struct {
+  int nppe;             /* particles of species 0 per element (ParticleFill, particle_fill.hpp:59) */
+  double temp;          /* fill temperature (2 vth^2) */
+  double density;       /* fill density; also the uniform neutralising background */
+  double pert_amp;      /* weights scaled by 1 + pert_amp sin(2 pi (x - x_lo)/L_x) */
+  double drift;         /* two counter-drifting halves at +-drift along x (0: one Maxwellian) */
+  uint64_t seed_offset; /* 0 = the reference's seeds */
+} mpic_batch_fill_desc;
+
+typedef struct {
+  int push_grid;        /* workgroups of the push (particles per workgroup = particles / push_grid) */
+  int solve_threads;    /* threads per problem in the solve (CSR rows per wavefront = rows / (threads / 64)) */
+  int tuned;            /* 1 after mpic_batch_tune */
+  int graph;            /* 1 while steps replay captured HIP graphs */
+  int max_inner_iters;  /* largest inner iteration count over the problems in the last step */
+  int all_converged;
+  int64_t num_particles;
+  double ms_step;       /* device time of the last step */
+} mpic_batch_stats;
+
+/* Builds the ensemble from host meshes (mpic_mesh_brick), borrowed for the call. */
+int mpic_batch_create(int nprob, mpic_mesh* const* meshes, const mpic_config* cfg, mpic_batch** out);
+void mpic_batch_destroy(mpic_batch* b);
+/* Fills every problem (fills[nprob]) and deposits the initial charge. */
+int mpic_batch_fill(mpic_batch* b, const mpic_batch_fill_desc* fills);
+/* Times the candidate tile shapes on the current state (nothing is advanced) and keeps the fastest. */
+int mpic_batch_tune(mpic_batch* b, double dt);
+/* One PIC::time_step of every problem.  With graphs enabled the step is a replay. */
+int mpic_batch_step(mpic_batch* b, double dt, mpic_batch_stats* st);
+/* enable = 1: capture the step (dt of the next call) and replay it from then on. */
+int mpic_batch_graph(mpic_batch* b, int enable);
+/* Per-problem results of the last step: kinetic energy, inner iterations, outer
+ * relative residual, converged flag, particle count (arrays of nprob; any may be NULL). */
+int mpic_batch_problem_stats(mpic_batch* b, double* ke, int* iters, double* relres, int* converged, int64_t* nparts);
+int mpic_batch_get_phi(mpic_batch* b, int prob, double* phi);   /* the problem's node count of entries */
+int mpic_batch_get_rhs(mpic_batch* b, int prob, double* rhs);
+/* The problem's particles (element ids local to the problem). */
+int mpic_batch_download(mpic_batch* b, int prob, mpic_soa_host* h, int64_t* n);
+int mpic_batch_check_ownership(mpic_batch* b, int64_t* mismatches);
+const char* mpic_batch_last_error(const mpic_batch* b);
 
 /* ---- instrumentation (PIC::output_timers, PIC.hpp:85-88) --------------------- */
 /* Average device duration (ms) of the last launch of the named kernel family:

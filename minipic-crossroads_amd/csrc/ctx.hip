@@ -220,6 +220,20 @@ int do_weight_charge(mpic_ctx* c) {
   return MPIC_OK;
 }
 
+int finish_solve(mpic_ctx* c, int* iters, double* relres);
+
+// Buffers of the persistent solves and the resident-grid capacity (once).
+int persist_prepare(mpic_ctx* c) {
+  if (c->persist_cap >= 0) return MPIC_OK;
+  int rc;
+  if ((rc = dalloc(c, &c->sync_epoch, (size_t)c->cg.nchunks + 2)) || (rc = dalloc(c, &c->sync_err, 2))) return rc;
+  if (hipHostMalloc((void**)&c->sync_err_h, sizeof(int)) != hipSuccess)
+    return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+  c->persist_cap = persist_capacity(0);
+  c->persist_cap32 = persist_capacity(1);
+  return MPIC_OK;
+}
+
 int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   c->cg.x = c->phi;
   c->cg.b = c->rhs;
@@ -228,6 +242,26 @@ int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
     c->cg.rho_hist = nullptr;
     HIPCHK(c, hipMalloc((void**)&c->cg.rho_hist, sizeof(double) * (size_t)(max_iter + 1)));
     c->rho_hist_cap = max_iter + 1;
+  }
+  // One cooperative launch for the whole solve when the grid is resident at once.
+  if (c->cfg.persistent_solve && !c->persist_off) {
+    int rc = persist_prepare(c);
+    if (rc) return rc;
+    if (c->persist_cap >= c->cg.nchunks) {
+      HIPCHK(c, launch_cg_setup(c->cg, tol, max_iter, c->stream));
+      HIPCHK(c, launch_zero(c->sync_epoch, ((size_t)c->cg.nchunks + 2) / 2 * 8, c->stream));
+      HIPCHK(c, launch_zero(c->sync_err, 8, c->stream));
+      const hipError_t e = launch_cg_persist(c->m, c->cg, tol, max_iter, c->sync_epoch, c->sync_err, c->stream);
+      if (e == hipSuccess) {
+        HIPCHK(c, hipMemcpyAsync(c->cg_h, c->cg.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->sync_err_h, c->sync_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (*c->sync_err_h == 0 && c->cg_h->done) return finish_solve(c, iters, relres);
+      } else {
+        (void)hipGetLastError();
+      }
+      c->persist_off = true;  // the per-iteration kernels from here on (x is unchanged)
+    }
   }
   HIPCHK(c, launch_cg_init(c->m, c->cg, tol, max_iter, c->stream));
   // Launch iterations in batches sized from the previous solve; kernels after
@@ -242,6 +276,54 @@ int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
     if (c->cg_h->done || it > max_iter) break;
     batch = 8;
   }
+  return finish_solve(c, iters, relres);
+}
+
+// BiCGStab_solve (CG_solve.hpp:115-206) from x = 0.  status 0 converged, 1
+// stopped at max_iter, 2 early exit (rho, omega or t.t zero), as the reference.
+int do_bicgstab(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres, int* status) {
+  BiBuffers& b = c->bi;
+  const int32_t nn = c->m.nn;
+  const int nch = c->cg.nchunks;
+  if (!b.st) {
+    int rc;
+    if ((rc = dalloc(c, &b.rn, nn)) || (rc = dalloc(c, &b.t, nn)) || (rc = dalloc(c, &b.v0, nn)) ||
+        (rc = dalloc(c, &b.v1, nn)) || (rc = dalloc(c, &b.part_bb, nch)) || (rc = dalloc(c, &b.part_r, 2 * nch)) ||
+        (rc = dalloc(c, &b.part_bv, nch)) || (rc = dalloc(c, &b.part_rt, 3 * nch)) || (rc = dalloc(c, &b.st, 1)))
+      return rc;
+    if (hipHostMalloc((void**)&c->bi_h, sizeof(BiState)) != hipSuccess)
+      return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+    b.nchunks = nch;
+  }
+  if (max_iter + 1 > c->bi_hist_cap) {
+    if (b.hist) (void)hipFree(b.hist);
+    b.hist = nullptr;
+    HIPCHK(c, hipMalloc((void**)&b.hist, sizeof(double) * 3 * (size_t)(max_iter + 1)));
+    c->bi_hist_cap = max_iter + 1;
+  }
+  b.x = c->phi;
+  b.b = c->rhs;
+  b.r = c->cg.r;
+  b.p0 = c->cg.p0;
+  b.p1 = c->cg.p1;
+  HIPCHK(c, launch_bi_init(c->m, b, tol, max_iter, c->stream));
+  int it = 0;
+  while (true) {
+    const int upto = std::min(max_iter + 1, it + 16);
+    for (; it < upto; ++it) HIPCHK(c, launch_bi_iter(c->m, b, it, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bi_h, b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->bi_h->done || it > max_iter) break;
+  }
+  const BiState& s = *c->bi_h;
+  c->last_cg_iters = s.iters;
+  if (iters) *iters = s.iters;
+  if (relres) *relres = s.rho0 > 0 ? std::sqrt(s.resid / s.rho0) : 0.0;
+  if (status) *status = s.done ? s.status : 1;
+  return MPIC_OK;
+}
+
+int finish_solve(mpic_ctx* c, int* iters, double* relres) {
   const CGState& s = *c->cg_h;
   c->last_cg_iters = s.iters;
   if (iters) *iters = s.iters;
@@ -265,6 +347,7 @@ void mpic_config_default(mpic_config* cfg) {
   cfg->kill_on_wall = 0;
   cfg->sort_period = 1;
   cfg->sort_bucket = 1;
+  cfg->persistent_solve = 0;
   cfg->num_species = 3;
   for (int t = 0; t < MPIC_MAX_SPECIES; ++t) cfg->species_m[t] = 1.0;
   cfg->species_q[0] = -1.0;  // e-  (particle_type.hpp:69-71)
@@ -308,7 +391,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   c->m.nn = nn;
   c->m.ne = ne;
   // Per-element records.
-  std::vector<double> geo((size_t)ne * 12), sixodet(ne);
+  std::vector<double> geo((size_t)ne * 12), sixodet(ne), erec((size_t)ne * 16, 0.0);
   std::vector<int32_t> conn((size_t)ne * 4), nbrs((size_t)ne * 4);
   for (int64_t e = 0; e < ne; ++e) {
     std::memcpy(&geo[e * 12], d->inv_jacobian + e * 9, 9 * sizeof(double));
@@ -323,6 +406,10 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       conn[e * 4 + k] = v;
       nbrs[e * 4 + k] = b;
     }
+    // walk record: J^-1 (9), node 0 (3), neighbours (4 x int32 in slots 12-13), 6/detJ
+    std::memcpy(&erec[e * 16], &geo[e * 12], 12 * sizeof(double));
+    std::memcpy(&erec[e * 16 + 12], &nbrs[e * 4], 4 * sizeof(int32_t));
+    erec[e * 16 + 14] = sixodet[e];
   }
   // Node -> element slot adjacency (slot = e*4 + k), element order.
   std::vector<int32_t> adj_ptr(nn + 1, 0), adj((size_t)ne * 4);
@@ -365,6 +452,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       }
     }
   double* dgeo;
+  double* derec;
   double* dgrad;
   double* dnormals;
   double* dsix;
@@ -380,6 +468,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   if ((rc = dupload(c, &ptr, vec, count_))) return bail(rc);
   size_t count_;
   count_ = geo.size(); UP(dgeo, geo.data());
+  count_ = erec.size(); UP(derec, erec.data());
   count_ = (size_t)ne * 12; UP(dgrad, d->grad_basis);
   count_ = (size_t)ne * 12; UP(dnormals, d->normals);
   count_ = (size_t)ne; UP(dsix, sixodet.data());
@@ -397,6 +486,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   count_ = (size_t)nn; UP(dbnd, d->boundary);
 #undef UP
   c->m.geo = dgeo;
+  c->m.erec = derec;
   c->m.grad = dgrad;
   c->m.normals = dnormals;
   c->m.sixodet = dsix;
@@ -445,6 +535,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   if ((rc = dupload(c, &c->species, sp, 48))) return bail(rc);
   // Fields, CG and sort buffers.
   const int nchunks = (nn + BLOCK - 1) / BLOCK;
+  if ((rc = dalloc(c, &c->m.ephi, (size_t)ne * 4))) return bail(rc);
   if ((rc = dalloc(c, &c->phi, nn)) || (rc = dalloc(c, &c->rhs, nn)) || (rc = dalloc(c, &c->acc, (size_t)ne * 16)) ||
       (rc = dalloc(c, &c->cg.r, nn)) || (rc = dalloc(c, &c->cg.p0, nn)) || (rc = dalloc(c, &c->cg.p1, nn)) ||
       (rc = dalloc(c, &c->cg.q, nn)) || (rc = dalloc(c, &c->cg.part_bb, nchunks)) ||
@@ -467,6 +558,10 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   if (c->cfg.form == MPIC_FORM_MIXED && (rc = mixed_prepare(c))) return bail(rc);
   if (c->cfg.form != MPIC_FORM_FP64 && c->cfg.form != MPIC_FORM_MIXED)
     return bail(fail(c, MPIC_ERR_ARG, "mpic_create: unknown form"));
+  if (c->cfg.solver != MPIC_SOLVER_CG && c->cfg.solver != MPIC_SOLVER_BICGSTAB)
+    return bail(fail(c, MPIC_ERR_ARG, "mpic_create: unknown solver"));
+  if (c->cfg.solver == MPIC_SOLVER_BICGSTAB && c->cfg.form == MPIC_FORM_MIXED)
+    return bail(fail(c, MPIC_ERR_ARG, "mpic_create: BiCGStab runs in the FP64 form only"));
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MPIC_ERR_HIP, "setup synchronisation failed"));
   *out = c;
   return MPIC_OK;
@@ -482,6 +577,9 @@ void mpic_destroy(mpic_ctx* c) {
   if (c->finish_h) (void)hipHostFree(c->finish_h);
   if (c->ir_h) (void)hipHostFree(c->ir_h);
   if (c->bad_h) (void)hipHostFree(c->bad_h);
+  if (c->bi_h) (void)hipHostFree(c->bi_h);
+  if (c->bi.hist) (void)hipFree(c->bi.hist);
+  if (c->sync_err_h) (void)hipHostFree(c->sync_err_h);
   if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
   if (c->ir.alpha_hist) (void)hipFree(c->ir.alpha_hist);
   if (c->start_h) (void)hipHostFree(c->start_h);
@@ -494,6 +592,8 @@ void mpic_destroy(mpic_ctx* c) {
 int mpic_set_form(mpic_ctx* c, int form) {
   if (!c) return MPIC_ERR_ARG;
   if (form != MPIC_FORM_FP64 && form != MPIC_FORM_MIXED) return fail(c, MPIC_ERR_ARG, "mpic_set_form: unknown form");
+  if (form == MPIC_FORM_MIXED && c->cfg.solver == MPIC_SOLVER_BICGSTAB)
+    return fail(c, MPIC_ERR_ARG, "mpic_set_form: BiCGStab runs in the FP64 form only");
   int rc;
   if (form == MPIC_FORM_MIXED && (rc = mixed_prepare(c))) return rc;
   if ((rc = mixed_state(c, form == MPIC_FORM_MIXED))) return rc;
@@ -643,6 +743,9 @@ int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel
   a.m = c->m;
   a.P = c->P[c->cur];
   a.offset = c->n;
+  a.e0 = 0;
+  a.ecount = c->m.ne;
+  a.lo0 = c->m.lo[0];
   a.nppe = nppe;
   a.type = type;
   a.density = density;
@@ -676,8 +779,17 @@ int mpic_weight_charge(mpic_ctx* c) {
 int mpic_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   if (!c || !(tol >= 0) || max_iter < 0) return fail(c, MPIC_ERR_ARG, "mpic_solve: bad arguments");
   tstart(c, T_SOLVE);
-  int rc = c->cfg.form == MPIC_FORM_MIXED ? mixed_solve(c, tol, max_iter, iters, relres)
-                                         : do_solve(c, tol, max_iter, iters, relres);
+  int rc;
+  if (c->cfg.form == MPIC_FORM_MIXED) {
+    rc = mixed_solve(c, tol, max_iter, iters, relres);
+  } else if (c->cfg.solver == MPIC_SOLVER_BICGSTAB) {
+    int status = 0;
+    rc = do_bicgstab(c, tol, max_iter, iters, relres, &status);
+    if (rc == MPIC_OK && status != 0)
+      rc = fail(c, MPIC_ERR_NOT_CONVERGED, status == 1 ? "BiCGStab reached max_iter" : "BiCGStab stopped early");
+  } else {
+    rc = do_solve(c, tol, max_iter, iters, relres);
+  }
   tstop(c, T_SOLVE);
   (void)hipStreamSynchronize(c->stream);
   c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
@@ -690,6 +802,7 @@ int mpic_weight_efield(mpic_ctx* c) {
   if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_weight_efield: " FP64_ONLY);
   int rc = ensure_transients(c);
   if (rc) return rc;
+  HIPCHK(c, launch_elem_phi(c->m, c->phi, c->stream));
   HIPCHK(c, launch_gather(push_args(c), c->stream));
   return MPIC_OK;
 }
@@ -768,9 +881,17 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   tstop(c, T_DEPOSIT);
   // 2. field solve, warm start from the previous potential (PIC.cpp:251)
   tstart(c, T_SOLVE);
-  rc = do_solve(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres);
+  if (c->cfg.solver == MPIC_SOLVER_BICGSTAB) {
+    // PIC::time_step ignores the solver's outcome (PIC.cpp:251): only a failed
+    // launch stops the step; the residual is reported in the statistics
+    int status = 0;
+    rc = do_bicgstab(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres, &status);
+  } else {
+    rc = do_solve(c, c->cfg.cg_tol, c->cfg.cg_max_iter, &st.cg_iters, &st.cg_relres);
+  }
   tstop(c, T_SOLVE);
   if (rc) return rc;
+  HIPCHK(c, launch_elem_phi(c->m, c->phi, c->stream));
   // 3. fused gather + kick + walk + deposit of the new positions + KE (PIC.cpp:252-258);
   //    with a pending ranking the push reads through it and writes element order
   if (!(c->ke_max_prev > 0) && c->n > 0) {

@@ -50,11 +50,11 @@ void record_times(mpic_ctx* c) {
 int mixed_prepare(mpic_ctx* c) {
   if (!c->mixed_ready) {
     const int32_t nn = c->m.nn, ne = c->m.ne;
-    float* geo32;
+    float* erec32;
     float* ell32;
     float4* E32;
     int rc;
-    if ((rc = dalloc(c, &geo32, (size_t)ne * 12)) || (rc = dalloc(c, &ell32, (size_t)c->ell_entries)) ||
+    if ((rc = dalloc(c, &erec32, (size_t)ne * 16)) || (rc = dalloc(c, &ell32, (size_t)c->ell_entries)) ||
         (rc = dalloc(c, &E32, (size_t)ne)) || (rc = dalloc(c, &c->acc64, (size_t)ne)) ||
         (rc = dalloc(c, &c->ir.r, nn)) || (rc = dalloc(c, &c->ir.d, nn)) || (rc = dalloc(c, &c->ir.p, nn)) ||
         (rc = dalloc(c, &c->ir.rv[0], nn)) || (rc = dalloc(c, &c->ir.rv[1], nn)) ||
@@ -65,7 +65,7 @@ int mixed_prepare(mpic_ctx* c) {
         (rc = dalloc(c, &c->bad, 1)))
       return rc;
     if (!c->background.empty() && (rc = dupload(c, &c->bg64, c->background.data(), c->background.size()))) return rc;
-    HIPCHK(c, launch_to_float(c->m.geo, geo32, (size_t)ne * 12, c->stream));
+    HIPCHK(c, launch_erec32(c->m, erec32, c->stream));
     HIPCHK(c, launch_to_float(c->m.ell_val, ell32, (size_t)c->ell_entries, c->stream));
     HIPCHK(c, launch_zero(c->acc64, (size_t)ne * 8, c->stream));
     HIPCHK(c, launch_zero(c->ir.st, sizeof(IRState), c->stream));
@@ -74,7 +74,7 @@ int mixed_prepare(mpic_ctx* c) {
     if (!c->bad_h && hipHostMalloc((void**)&c->bad_h, sizeof(unsigned long long)) != hipSuccess)
       return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
     c->ir.nchunks = c->cg.nchunks;
-    c->m32.geo = geo32;
+    c->m32.erec = erec32;
     c->m32.E = E32;
     c->m32.ell_val = ell32;
     c->mixed_ready = true;
@@ -156,6 +156,34 @@ int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relre
     HIPCHK(c, hipMalloc((void**)&c->ir.rho_hist, sizeof(double) * (size_t)(max_iter + 1)));
     HIPCHK(c, hipMalloc((void**)&c->ir.alpha_hist, sizeof(double) * (size_t)(max_iter + 1)));
     c->ir_hist_cap = max_iter + 1;
+  }
+  // One cooperative launch for the whole refinement solve when the grid is
+  // resident at once; the same arithmetic as the per-iteration kernels below.
+  if (c->cfg.persistent_solve && !c->persist_off) {
+    if ((rc = persist_prepare(c))) return rc;
+    if (c->persist_cap32 >= c->ir.nchunks) {
+      HIPCHK(c, launch_ir_setup(c->ir, tol, max_iter, c->stream));
+      HIPCHK(c, launch_zero(c->sync_epoch, ((size_t)c->ir.nchunks + 2) / 2 * 8, c->stream));
+      HIPCHK(c, launch_zero(c->sync_err, 8, c->stream));
+      const hipError_t e = launch_ir_persist(c->m, c->m32, c->ir, tol, max_iter, IR_IN_TOL_MIN, c->sync_epoch,
+                                             c->sync_err, c->stream);
+      if (e == hipSuccess) {
+        HIPCHK(c, hipMemcpyAsync(c->ir_h, c->ir.st, sizeof(IRState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->sync_err_h, c->sync_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (*c->sync_err_h == 0 && c->ir_h->done) {
+          const IRState& s = *c->ir_h;
+          c->last_cg_iters = s.total_inner;
+          if (iters) *iters = s.total_inner;
+          if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+          if (!s.converged) return fail(c, MPIC_ERR_NOT_CONVERGED, "refinement did not converge");
+          return MPIC_OK;
+        }
+      } else {
+        (void)hipGetLastError();
+      }
+      c->persist_off = true;  // the per-iteration kernels from here on
+    }
   }
   std::vector<int> passes;
   bool first = true;

@@ -258,6 +258,11 @@ hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_it
   return hipGetLastError();
 }
 
+hipError_t launch_cg_setup(CGBuffers& c, double tol, int max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(1), 0, s, c.st, tol * tol, max_iter);
+  return hipGetLastError();
+}
+
 hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s) {
   const CGArgs a = make_args(m, c, it);
   hipLaunchKernelGGL(k_cg_k1, dim3(c.nchunks), dim3(BLOCK), 0, s, a);

@@ -22,8 +22,6 @@
 namespace mpic {
 namespace {
 
-constexpr int IR_MAX_OUTER = 40;
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off);
@@ -166,7 +164,7 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
     const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
     const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
     float w = 0;
-    for (int k = 0; k < len; ++k) w += vp[k * 64] * (float)(a.r[cp[k * 64]] * inv);
+    for (int k = 0; k < len; ++k) w = fmaf(vp[k * 64], (float)(a.r[cp[k * 64]] * inv), w);
     const float ri = (float)(a.r[i] * inv);
     a.r_nxt[i] = ri;  // r_0 and w_0 live in the parity-0 buffers
     a.w_nxt[i] = w;
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_ir_cg(IRArgs a) {
 __global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a) {
   if (a.st->done) return;
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i < a.m.nn) a.x[i] = a.x[i] + a.st->scale * (double)a.d[i];
+  if (i < a.m.nn) a.x[i] = fma(a.st->scale, (double)a.d[i], a.x[i]);
 }
 
 // Arguments of inner iteration it (k_ir_start fills the parity-0 buffers: it = -1).
@@ -333,6 +331,12 @@ hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bo
 
 hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s) {
   hipLaunchKernelGGL(k_ir_cg, dim3(c.nchunks), dim3(BLOCK), 0, s, make_args(m, m32, c, it));
+  return hipGetLastError();
+}
+
+hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s) {
+  if (!c.st) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ir_setup, dim3(1), dim3(1), 0, s, c.st, tol * tol, max_iter);
   return hipGetLastError();
 }
 

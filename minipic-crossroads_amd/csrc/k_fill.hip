@@ -1,5 +1,8 @@
 // Device particle fill: ParticleFill / FillFunctor (particle_fill.hpp:59-164)
-// with one thread per (element, ipart).  Particle index = offset + e*nppe + ipart.
+// with one thread per (element, ipart) over the elements [e0, e0 + ecount)
+// (the whole mesh, or one problem of an ensemble).  Particle index = offset +
+// (e - e0)*nppe + ipart; the seed formula takes the element index and element
+// count of that range.
 //   seed = elem + (34030+ipart)*Nglob + Nglob*12345*type   (u64 wrap, particle_fill.hpp:147)
 //   10 burned draws; reference position by rejection (sum > 1 rejected);
 //   x = refToPhysical; velocity by set_vel (particle_fill.hpp:101-124);
@@ -24,12 +27,13 @@ struct Lcg {
 
 __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
   const int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const int64_t total = (int64_t)a.m.ne * a.nppe;
+  const int64_t total = (int64_t)a.ecount * a.nppe;
   if (q >= total) return;
-  const int e = (int)(q / a.nppe);
-  const int ip = (int)(q - (int64_t)e * a.nppe);
-  const unsigned long long N = (unsigned long long)a.m.ne;
-  Lcg r{(unsigned long long)e + (34030ull + (unsigned long long)ip) * N + N * 12345ull * (unsigned long long)a.type};
+  const int el = (int)(q / a.nppe);  // element within the filled range
+  const int ip = (int)(q - (int64_t)el * a.nppe);
+  const int e = a.e0 + el;
+  const unsigned long long N = (unsigned long long)a.ecount;
+  Lcg r{(unsigned long long)el + (34030ull + (unsigned long long)ip) * N + N * 12345ull * (unsigned long long)a.type};
   r.s += a.seed_offset * 0x9E3779B97F4A7C15ull;
   for (int i = 0; i < 10; ++i) r.next();
   double ref[3];
@@ -73,7 +77,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
   }
   const double vol = a.m.detj[e] / 6.;
   double wp = a.density * vol / a.nppe;
-  if (a.pert_amp != 0.0) wp = wp * (1.0 + a.pert_amp * sin(a.pert_k * (ph[0] - a.m.lo[0])));
+  if (a.pert_amp != 0.0) wp = wp * (1.0 + a.pert_amp * sin(a.pert_k * (ph[0] - a.lo0)));
   const int64_t p = a.offset + q;
   Rec64 o;
   for (int d = 0; d < 3; ++d) {
@@ -93,8 +97,9 @@ __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
 }  // namespace
 
 hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
-  const int64_t total = (int64_t)a.m.ne * a.nppe;
+  const int64_t total = (int64_t)a.ecount * a.nppe;
   if (total <= 0) return hipSuccess;
+  if (a.e0 < 0 || (int64_t)a.e0 + a.ecount > a.m.ne) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((total + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }

@@ -26,31 +26,59 @@ constexpr float LOC_EPS = 2e-4f;
 constexpr int LOC_CAP = 16;        // element moves of one point location
 constexpr double OWNER_TOL = 1e-9; // ownership check: tolerated negative coordinate
 
-__device__ __forceinline__ void load12f(const float* __restrict__ base, int e, float o[12]) {
-  const float4* q = reinterpret_cast<const float4*>(base + (size_t)e * 12);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float4 t = q[i];
-    o[4 * i + 0] = t.x;
-    o[4 * i + 1] = t.y;
-    o[4 * i + 2] = t.z;
-    o[4 * i + 3] = t.w;
-  }
+// The walk's FP32 element record (64 B): inverse Jacobian, node 0 and the face
+// neighbours, so a crossing needs one load.
+struct ElemRec32 {
+  float J[9];
+  float n0[3];
+  int nb[4];
+};
+
+__device__ __forceinline__ ElemRec32 load_erec32(const float* __restrict__ erec, int e) {
+  const float4* q = reinterpret_cast<const float4*>(erec + (size_t)e * 16);
+  const float4 a0 = q[0], a1 = q[1], a2 = q[2];
+  const int4 a3 = reinterpret_cast<const int4*>(erec + (size_t)e * 16)[3];
+  ElemRec32 r;
+  r.J[0] = a0.x;
+  r.J[1] = a0.y;
+  r.J[2] = a0.z;
+  r.J[3] = a0.w;
+  r.J[4] = a1.x;
+  r.J[5] = a1.y;
+  r.J[6] = a1.z;
+  r.J[7] = a1.w;
+  r.J[8] = a2.x;
+  r.n0[0] = a2.y;
+  r.n0[1] = a2.z;
+  r.n0[2] = a2.w;
+  r.nb[0] = a3.x;
+  r.nb[1] = a3.y;
+  r.nb[2] = a3.z;
+  r.nb[3] = a3.w;
+  return r;
 }
 
-// FP64 point location of the FP32 position X (widened exactly), starting in e:
+// FP64 point location of the FP32 position x (widened exactly), starting in e:
 // moves across the face of the most negative barycentric coordinate until none
 // is negative, a wall is reached, the walk would step back, or LOC_CAP moves.
-// R becomes the reference position in the final element.  Face of each
-// coordinate: lambda_0 = 1 - sum R -> face 1, R0 -> face 2, R1 -> face 0, R2 -> face 3
-// (faces f0 = (0,1,3), f1 = (1,2,3), f2 = (0,3,2), f3 = (0,2,1)).
-__device__ __noinline__ void locate64(const DevMesh& m, int& e, const float X[3], float R[3], int& relocated) {
-  const double x[3] = {(double)X[0], (double)X[1], (double)X[2]};
+// Returns the final element, its reference position and the number of moves.
+// Face of each coordinate: lambda_0 = 1 - sum R -> face 1, R0 -> face 2,
+// R1 -> face 0, R2 -> face 3 (faces f0 = (0,1,3), f1 = (1,2,3), f2 = (0,3,2),
+// f3 = (0,2,1)).  Arguments and result travel in registers: the cold FP64 path
+// stays out of line without putting the hot loop's state in scratch memory.
+struct Located {
+  float r0, r1, r2;
+  int e, moves;
+};
+
+__device__ __noinline__ Located locate64(const double* __restrict__ geo, const int4* __restrict__ nbrs, int e,
+                                         float X0, float X1, float X2) {
+  const double x[3] = {(double)X0, (double)X1, (double)X2};
   double r[3];
-  int prev = -1;
+  int prev = -1, moves = 0;
   for (int k = 0;; ++k) {
     double g[12];
-    load12(m.geo, e, g);
+    load12(geo, e, g);
     const double dl[3] = {x[0] - g[9], x[1] - g[10], x[2] - g[11]};
     p2r_vec(g, dl, r);
     const double lam[4] = {1.0 - r[0] - r[1] - r[2], r[0], r[1], r[2]};
@@ -59,16 +87,14 @@ __device__ __noinline__ void locate64(const DevMesh& m, int& e, const float X[3]
     for (int i = 1; i < 4; ++i)
       if (lam[i] < lam[j]) j = i;
     if (!(lam[j] < 0.0) || k == LOC_CAP) break;
-    const int4 nb = m.nbrs[e];
+    const int4 nb = nbrs[e];
     const int e2 = j == 0 ? nb.y : j == 1 ? nb.z : j == 2 ? nb.x : nb.w;
     if (e2 < 0 || e2 == prev) break;
     prev = e;
     e = e2;
-    relocated++;
+    moves++;
   }
-  R[0] = (float)r[0];
-  R[1] = (float)r[1];
-  R[2] = (float)r[2];
+  return Located{(float)r[0], (float)r[1], (float)r[2], e, moves};
 }
 
 #pragma clang fp contract(fast)
@@ -84,8 +110,10 @@ __device__ __forceinline__ bool near_face(const float R[3]) {
 }
 
 // The tet walk of MoveParticle (particle_move.hpp:103-124,136-225) in FP32.
-__device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict__ geo, int& e, float X[3], float R[3],
-                                       float V[3], float VR[3], float dt, int kill, WalkCount& wc) {
+// g holds the record of element e on entry and of the final element on return.
+__device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict__ erec, int& e, ElemRec32& g,
+                                       float X[3], float R[3], float V[3], float VR[3], float dt, int kill,
+                                       WalkCount& wc) {
   float tr = 1.0f;
   int iters = 0;
   while (tr > 0.0f) {
@@ -134,15 +162,13 @@ __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict
       R[d] += sl * VR[d];
     }
     if (s < 1) {
-      const int4 nb = m.nbrs[e];
-      const int e2 = face == 0 ? nb.x : face == 1 ? nb.y : face == 2 ? nb.z : nb.w;
-      float g[12];
+      const int e2 = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
       if (e2 >= 0) {
         e = e2;
         wc.crossings++;
-        load12f(geo, e, g);
-        const float dl[3] = {X[0] - g[9], X[1] - g[10], X[2] - g[11]};
-        p2r_vecf(g, dl, R);
+        g = load_erec32(erec, e);
+        const float dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
+        p2r_vecf(g.J, dl, R);
       } else {
         if (kill) return false;
         wc.reflections++;
@@ -152,9 +178,8 @@ __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict
         V[0] -= 2 * vn * n0;
         V[1] -= 2 * vn * n1;
         V[2] -= 2 * vn * n2;
-        load12f(geo, e, g);
       }
-      p2r_vecf(g, V, VR);
+      p2r_vecf(g.J, V, VR);
     }
     if (!(s > 0)) wc.nonpos++;
     tr -= s * tr;
@@ -271,8 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
         t = (int)rc.type;
         w = rc.w;
         const float4 E = a.m32.E[e];
-        float g[12];
-        load12f(a.m32.geo, e, g);
+        ElemRec32 g = load_erec32(a.m32.erec, e);
         // Boris push with B = 0 (two half kicks), particle_move.hpp:364-392
         const float hq = a.dt * a.scale * (float)a.species[32 + t] * 0.5f;
         const float Ev[3] = {E.x, E.y, E.z};
@@ -282,9 +306,16 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
           V[d] += hq * Ev[d];
         }
         float VR[3];
-        p2r_vecf(g, V, VR);
-        const bool alive = walk32(a.m, a.m32.geo, e, X, R, V, VR, a.dt, a.kill, wc);
-        if (alive && near_face(R)) locate64(a.m, e, X, R, wc.relocated);
+        p2r_vecf(g.J, V, VR);
+        const bool alive = walk32(a.m, a.m32.erec, e, g, X, R, V, VR, a.dt, a.kill, wc);
+        if (alive && near_face(R)) {
+          const Located L = locate64(a.m.geo, a.m.nbrs, e, X[0], X[1], X[2]);
+          e = L.e;
+          R[0] = L.r0;
+          R[1] = L.r1;
+          R[2] = L.r2;
+          wc.relocated += L.moves;
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           rc.x[d] = X[d];
@@ -430,6 +461,18 @@ __global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticle
   }
 }
 
+// FP32 walk records from the FP64 ones: J^-1 and node 0 rounded, neighbours copied.
+__global__ __launch_bounds__(BLOCK) void k_erec32(DevMesh m, float* __restrict__ out) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= m.ne) return;
+  const double* src = m.erec + (size_t)e * 16;
+  float4* q = reinterpret_cast<float4*>(out + (size_t)e * 16);
+  q[0] = make_float4((float)src[0], (float)src[1], (float)src[2], (float)src[3]);
+  q[1] = make_float4((float)src[4], (float)src[5], (float)src[6], (float)src[7]);
+  q[2] = make_float4((float)src[8], (float)src[9], (float)src[10], (float)src[11]);
+  reinterpret_cast<int4*>(q)[3] = reinterpret_cast<const int4*>(src)[6];
+}
+
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = (float)src[i];
@@ -471,10 +514,11 @@ hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStr
 }
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  if (!a.m32.geo || !a.m32.E || !a.P.r || !a.Q.r || !a.P.elem || !a.Q.elem || !a.acc || !a.partial || !a.species ||
+  if (!a.m32.erec || !a.m32.E || !a.P.r || !a.Q.r || !a.P.elem || !a.Q.elem || !a.acc || !a.partial || !a.species ||
       !a.m.bucket)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_push32, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, a);
+  const unsigned g = a.grid > 0 ? (unsigned)(a.grid < MAX_PARTIALS ? a.grid : MAX_PARTIALS) : grid_stride(a.n);
+  hipLaunchKernelGGL(k_push32, dim3(g), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_deposit32(const DevMesh& m, const DevParticles32& P, int64_t n, const double* species, int S,
@@ -510,6 +554,11 @@ hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, in
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_convert, dim3(grid_for(n)), dim3(BLOCK), 0, s, P64, P32, off, n, to32 ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s) {
+  if (m.ne <= 0 || !m.erec || !out) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_erec32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, out);
   return hipGetLastError();
 }
 hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s) {

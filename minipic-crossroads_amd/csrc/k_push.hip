@@ -14,13 +14,50 @@
 namespace mpic {
 namespace {
 
+// The walk's element record (one 128-B line): inverse Jacobian, node 0, face
+// neighbours and 6/detJ.  A crossing needs one load: the neighbours of the new
+// element arrive with its Jacobian.
+struct ElemRec {
+  double J[9];
+  double n0[3];
+  int nb[4];
+  double sixodet;
+};
+
+__device__ __forceinline__ ElemRec load_erec(const double* __restrict__ erec, int e) {
+  const double2* q = reinterpret_cast<const double2*>(erec + (size_t)e * 16);
+  const double2 a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3], a4 = q[4], a5 = q[5], a7 = q[7];
+  const int4 a6 = reinterpret_cast<const int4*>(erec + (size_t)e * 16)[6];
+  ElemRec r;
+  r.J[0] = a0.x;
+  r.J[1] = a0.y;
+  r.J[2] = a1.x;
+  r.J[3] = a1.y;
+  r.J[4] = a2.x;
+  r.J[5] = a2.y;
+  r.J[6] = a3.x;
+  r.J[7] = a3.y;
+  r.J[8] = a4.x;
+  r.n0[0] = a4.y;
+  r.n0[1] = a5.x;
+  r.n0[2] = a5.y;
+  r.nb[0] = a6.x;
+  r.nb[1] = a6.y;
+  r.nb[2] = a6.z;
+  r.nb[3] = a6.w;
+  r.sixodet = a7.x;
+  return r;
+}
+
 // Gather, PIC.cpp:162-184: weights from getWeights (mesh.hpp:170-176),
-// E_d = sum_i sum_j (phi_i*w_j)*G[e][i]_d.
-__device__ __forceinline__ void gather_E(const DevMesh& m, const double* __restrict__ phi, int e, const double R[3],
-                                         double E[3]) {
+// E_d = sum_i sum_j (phi_i*w_j)*G[e][i]_d.  The element's nodal potentials come
+// from m.ephi (copied from phi after the solve), so the gradient and the
+// potentials are loaded together.
+__device__ __forceinline__ void gather_E(const DevMesh& m, int e, const double R[3], double E[3]) {
   const double wt[4] = {1 - R[0] - R[1] - R[2], R[0], R[1], R[2]};
-  const int4 nd = m.conn[e];
-  const double dof[4] = {phi[nd.x], phi[nd.y], phi[nd.z], phi[nd.w]};
+  const double2* fp = reinterpret_cast<const double2*>(m.ephi + (size_t)e * 4);
+  const double2 f01 = fp[0], f23 = fp[1];
+  const double dof[4] = {f01.x, f01.y, f23.x, f23.y};
   double G[12];
   load12(m.grad, e, G);
   E[0] = E[1] = E[2] = 0.0;
@@ -60,9 +97,10 @@ __device__ __forceinline__ void kick(const double* Ji, double V[3], const double
 
 
 // MoveParticle::operator() + find_intersection, particle_move.hpp:103-124,136-225.
+// g holds the record of element e on entry and of the final element on return.
 // Returns false when the particle is killed on a wall.
-__device__ __forceinline__ bool walk(const DevMesh& m, int& e, double X[3], double R[3], double V[3], double VR[3],
-                                     double dt, int kill, WalkCount& wc) {
+__device__ __forceinline__ bool walk(const DevMesh& m, int& e, ElemRec& g, double X[3], double R[3], double V[3],
+                                     double VR[3], double dt, int kill, WalkCount& wc) {
   double tr = 1.0;
   int iters = 0;
   while (tr > 0) {
@@ -112,21 +150,19 @@ __device__ __forceinline__ bool walk(const DevMesh& m, int& e, double X[3], doub
       R[d] = R[d] + s * ldt * VR[d];
     }
     if (s < 1) {
-      const int4 nb = m.nbrs[e];
-      const int e2 = face == 0 ? nb.x : face == 1 ? nb.y : face == 2 ? nb.z : nb.w;
+      const int e2 = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
       if (e2 >= 0) {
         e = e2;
         wc.crossings++;
-        double g[12];
-        load12(m.geo, e, g);  // inverse Jacobian + node 0 of the new element
+        g = load_erec(m.erec, e);  // inverse Jacobian, node 0 and neighbours of the new element
         // physToReference, mesh.hpp:197-211
-        const double dl[3] = {X[0] - g[9], X[1] - g[10], X[2] - g[11]};
+        const double dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
         double dr[3];
         R[0] = R[1] = R[2] = 0.0;
-        p2r_vec(g, dl, dr);
+        p2r_vec(g.J, dl, dr);
 #pragma unroll
         for (int d = 0; d < 3; ++d) R[d] += dr[d];
-        p2r_vec(g, V, VR);
+        p2r_vec(g.J, V, VR);
       } else {
         if (kill) return false;
         wc.reflections++;
@@ -139,9 +175,7 @@ __device__ __forceinline__ bool walk(const DevMesh& m, int& e, double X[3], doub
         V[0] -= 2 * vn * n0;
         V[1] -= 2 * vn * n1;
         V[2] -= 2 * vn * n2;
-        double g[12];
-        load12(m.geo, e, g);
-        p2r_vec(g, V, VR);
+        p2r_vec(g.J, V, VR);
       }
     }
     if (!(s > 0)) wc.nonpos++;
@@ -150,9 +184,11 @@ __device__ __forceinline__ bool walk(const DevMesh& m, int& e, double X[3], doub
   return true;
 }
 
-// Per-node charge of one particle, PIC.cpp:121-131, in fixed point.
-__device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double w, double q, int S, fx128 c[4]) {
-  const double tq = w * q * m.sixodet[e];
+// Per-node charge of one particle, PIC.cpp:121-131, in fixed point
+// (sixodet = 6/detJ of element e).
+__device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double sixodet, double w, double q, int S,
+                                                fx128 c[4]) {
+  const double tq = w * q * sixodet;
   double W[16];
   load16(m.wbv, e, W);
 #pragma unroll
@@ -162,6 +198,16 @@ __device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double 
     for (int cc = 0; cc < 4; ++cc) acc += -tq * W[i * 4 + cc];
     c[i] = to_fixed(acc, S);
   }
+}
+
+// ephi[e][k] = phi[conn[e][k]] (after every solve).
+__global__ __launch_bounds__(BLOCK) void k_elem_phi(DevMesh m, const double* __restrict__ phi) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= m.ne) return;
+  const int4 nd = m.conn[e];
+  double2* out = reinterpret_cast<double2*>(m.ephi + (size_t)e * 4);
+  out[0] = double2{phi[nd.x], phi[nd.y]};
+  out[1] = double2{phi[nd.z], phi[nd.w]};
 }
 
 
@@ -200,6 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     double w = 0, V[3] = {0, 0, 0};
     int t = 0;
     if (p < hi) {
+      ElemRec g;
       const int64_t src = gather ? (int64_t)a.perm[p] : p;  // sorted order: gather through the permutation
       Rec64 rc = load_rec(a.P.r, src);
       int e = rc.elem;
@@ -213,13 +260,12 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         }
         t = (int)rc.type;
         w = rc.w;
+        g = load_erec(a.m.erec, e);
         double E[3];
-        gather_E(a.m, a.phi, e, R, E);
-        double g[12];
-        load12(a.m.geo, e, g);
+        gather_E(a.m, e, R, E);
         double VR[3];
-        kick(g, V, E, a.sp[32 + t], a.dt, a.scale, VR);
-        const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+        kick(g.J, V, E, a.sp[32 + t], a.dt, a.scale, VR);
+        const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           rc.x[d] = X[d];
@@ -238,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     }
     if (enew >= 0) {
       fx128 c[4];
-      particle_charge(a.m, enew, w, a.sp[t], a.S_charge, c);
+      particle_charge(a.m, enew, a.m.sixodet[enew], w, a.sp[t], a.S_charge, c);
       dep_table_add(table, a.acc, a.count, a.m.bucket, enew, c);
       const double term = ke_term(w, a.sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
@@ -260,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather(KArgs a) {
   if (e < 0) return;
   const int64_t cap = a.P.cap;
   double E[3];
-  gather_E(a.m, a.phi, e, rc.xr, E);
+  gather_E(a.m, e, rc.xr, E);
   for (int d = 0; d < 3; ++d) a.E[d * cap + p] = E[d];
 }
 
@@ -277,9 +323,8 @@ __global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
     V[d] = rc.v[d];
     E[d] = a.E[d * cap + p];
   }
-  double g[12];
-  load12(a.m.geo, e, g);
-  kick(g, V, E, a.sp[32 + rc.type], a.dt, a.scale, VR);
+  const ElemRec g = load_erec(a.m.erec, e);
+  kick(g.J, V, E, a.sp[32 + rc.type], a.dt, a.scale, VR);
   for (int d = 0; d < 3; ++d) {
     rc.v[d] = V[d];
     a.vr[d * cap + p] = VR[d];
@@ -305,7 +350,8 @@ __global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
       V[d] = rc.v[d];
       VR[d] = a.vr[d * cap + p];
     }
-    const bool alive = walk(a.m, e, X, R, V, VR, a.dt, a.kill, wc);
+    ElemRec g = load_erec(a.m.erec, e);
+    const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);
     for (int d = 0; d < 3; ++d) {
       rc.x[d] = X[d];
       rc.xr[d] = R[d];
@@ -335,7 +381,7 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(DevMesh m, DevParticles P, in
       const int e = rc.elem;
       if (e >= 0) {
         fx128 c[4];
-        particle_charge(m, e, rc.w, sp[rc.type], S, c);
+        particle_charge(m, e, m.sixodet[e], rc.w, sp[rc.type], S, c);
         dep_table_add(table, acc, nullptr, nullptr, e, c);
       }
     }
@@ -435,6 +481,11 @@ hipError_t launch_charge_bound(const DevParticles& P, int64_t n, const double* s
 hipError_t launch_push_fused(const PushArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_push_fused, dim3(grid_stride(a.n)), dim3(BLOCK), 0, s, kargs(a));
+  return hipGetLastError();
+}
+hipError_t launch_elem_phi(const DevMesh& m, const double* phi, hipStream_t s) {
+  if (m.ne <= 0 || !m.ephi) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_elem_phi, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, phi);
   return hipGetLastError();
 }
 hipError_t launch_gather(const PushArgs& a, hipStream_t s) {

@@ -62,8 +62,19 @@ struct mpic_ctx {
   unsigned long long* acc = nullptr;  // [ne*16] element charge digits
   CGBuffers cg{};
   CGState* cg_h = nullptr;  // pinned
+  // persistent solves: grid epochs and error flag (device), error flag (pinned),
+  // resident-grid capacity (-1 = not yet queried), disabled after a failed launch
+  unsigned* sync_epoch = nullptr;
+  int* sync_err = nullptr;
+  int* sync_err_h = nullptr;
+  int persist_cap = -1, persist_cap32 = 0;
+  bool persist_off = false;
   int last_cg_iters = 0;
   int rho_hist_cap = 0;
+  // BiCGStab (allocated on first use)
+  mpic::BiBuffers bi{};
+  mpic::BiState* bi_h = nullptr;  // pinned
+  int bi_hist_cap = 0;
 
   SortBuffers sb{};
   int32_t* start_h = nullptr;  // pinned [1]
@@ -143,6 +154,8 @@ inline double telapsed(mpic_ctx* c, int t) {
 int finish_partials(mpic_ctx* c, int parts, PushPartial* out);
 int materialize(mpic_ctx* c);
 int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live);
+// Buffers and capacities of the persistent solves (once per context).
+int persist_prepare(mpic_ctx* c);
 
 // ---- precision form (ctx_mixed.hip) ----
 // Allocates the precision-form tables and particle sets (idempotent).

@@ -15,7 +15,9 @@ namespace mpic {
 constexpr int BLOCK = 256;         // threads per workgroup for particle and row kernels
 constexpr int WALK_CAP = 1000;     // crossings per particle per step (the reference has no cap)
 constexpr int ELL_SLICE = 64;      // rows per ELL slice = one wavefront
-constexpr int MAX_PARTIALS = 4096; // workgroups of the grid-stride particle kernels
+constexpr int MAX_PARTIALS = 8192; // workgroups of the grid-stride particle kernels (upper bound)
+constexpr int DEFAULT_GRID = 4096; // their default number of workgroups
+constexpr int IR_MAX_OUTER = 40;   // refinement passes of the precision-form solve
 
 // 128-bit two's-complement fixed-point value (lo word first).
 struct fx128 {
@@ -29,6 +31,10 @@ struct fx128 {
 struct DevMesh {
   int32_t nn, ne;
   const double* geo;       // [ne][12]  inverse Jacobian (row-major 3x3) then node-0 coordinates
+  // [ne][16]  the walk's element record, one 128-B line: inverse Jacobian (9),
+  // node 0 (3), face neighbours (4 x int32 in 2 slots), 6/detJ, unused
+  const double* erec;
+  double* ephi;            // [ne][4]   potential at the element's nodes (refreshed after each solve)
   const double* grad;      // [ne][12]  physical P1 gradients G[e][i][d]
   const int4* conn;        // [ne]      element -> 4 node ids
   const int4* nbrs;        // [ne]      face f -> neighbour element, -1 wall
@@ -100,7 +106,7 @@ struct DevParticles32 {
 
 // Precision-form mesh tables.
 struct Mesh32 {
-  const float* geo;       // [ne][12] inverse Jacobian (row-major) then node 0, FP32
+  const float* erec;      // [ne][16] inverse Jacobian (9), node 0 (3), face neighbours (4 x int32), FP32
   float4* E;              // [ne]     field of the current potential (constant per tet)
   const float* ell_val;   // FP32 stiffness values in the sliced ELL layout of DevMesh
 };
@@ -180,7 +186,7 @@ __host__ __device__ inline long long to_fixed64(double v, int S) { return (long 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 inline unsigned grid_stride(int64_t n) {
   const int64_t g = (n + BLOCK - 1) / BLOCK;
-  return (unsigned)(g < MAX_PARTIALS ? g : MAX_PARTIALS);
+  return (unsigned)(g < DEFAULT_GRID ? g : DEFAULT_GRID);
 }
 
 
@@ -207,6 +213,7 @@ struct PushArgs {
 };
 
 hipError_t launch_push_fused(const PushArgs& a, hipStream_t s);   // gather+kick+walk+deposit+KE
+hipError_t launch_elem_phi(const DevMesh& m, const double* phi, hipStream_t s);  // m.ephi from phi
 hipError_t launch_gather(const PushArgs& a, hipStream_t s);       // weight_Efield -> E
 hipError_t launch_accel(const PushArgs& a, hipStream_t s);        // kick from E -> v, vr
 hipError_t launch_move(const PushArgs& a, hipStream_t s);         // walk from v, vr
@@ -233,8 +240,35 @@ struct CGBuffers {
   int nchunks;
 };
 hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_iter, hipStream_t s);
+// Resets the solver state (tolerance, max_iter, done = 0) without the first residual.
+hipError_t launch_cg_setup(CGBuffers& c, double tol, int max_iter, hipStream_t s);
+// The whole CG as one cooperative launch (k_cg_persist.hip); epoch [nchunks] and
+// err must be zero.  Results equal the per-iteration kernels bit for bit.
+hipError_t launch_cg_persist(const DevMesh& m, CGBuffers& c, double tol, int max_iter, unsigned* epoch, int* err,
+                             hipStream_t s);
+// Workgroups of a persistent solve kernel resident at once (0: no cooperative
+// launch); which = 0 FP64 CG, 1 precision-form refinement.
+int persist_capacity(int which);
 // Iteration it: convergence test + p/q update, then (it < max_iter) the x/r update.
 hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s);
+
+// BiCGStab of the reference (k_bicgstab.hip).
+struct BiState {
+  int done, iters, status, max_iter;  // status 0 converged, 1 max_iter, 2 early exit
+  double rho0, rho, resid, tol;
+};
+struct BiBuffers {
+  double *x, *b, *r, *rn, *t, *p0, *p1, *v0, *v1;
+  double* part_bb;  // [nchunks]   b.b
+  double* part_r;   // [2*nchunks] r.r and r~.r per chunk
+  double* part_bv;  // [nchunks]   b.v
+  double* part_rt;  // [3*nchunks] r'.r', t.t and t.r' per chunk
+  double* hist;     // [3*(max_iter+1)] rho, alpha, omega of each iteration
+  BiState* st;
+  int nchunks;
+};
+hipError_t launch_bi_init(const DevMesh& m, BiBuffers& c, double tol, int max_iter, hipStream_t s);
+hipError_t launch_bi_iter(const DevMesh& m, BiBuffers& c, int it, hipStream_t s);
 
 // ---- precision form (k_mixed.hip, k_cg32.hip) ----
 struct PushArgs32 {
@@ -250,6 +284,7 @@ struct PushArgs32 {
   unsigned long long* acc;  // [ne] element charge sums, 64-bit fixed point at 2^-S_charge
   PushPartial* partial;
   const double* species;
+  int grid;                 // workgroups (0: the default grid-stride size), at most MAX_PARTIALS
 };
 hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStream_t s);
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s);
@@ -268,6 +303,7 @@ hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst
 hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n, bool to32,
                           hipStream_t s);
 hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s);  // FP32 walk records
 // Ownership check: adds to *bad the live particles whose element does not
 // contain their position under the FP64 barycentric test.
 hipError_t launch_check_owner(const DevMesh& m, const DevParticles& P, int64_t n, unsigned long long* bad,
@@ -297,6 +333,11 @@ hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bo
 hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s);
 // x += |r| d (FP64).
 hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, hipStream_t s);
+hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s);
+// The whole precision-form refinement solve as one cooperative launch; same
+// results as launch_ir_outer / launch_ir_inner / launch_ir_update.
+hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, double tol, int max_iter,
+                             double in_tol_min, unsigned* epoch, int* err, hipStream_t s);
 
 struct SortBuffers {
   int32_t* count;    // [ne+1]
@@ -319,10 +360,31 @@ struct FillArgs {
   DevMesh m;
   DevParticles P;
   int64_t offset;
+  int32_t e0, ecount;  // filled elements [e0, e0 + ecount)
   int nppe, type;
-  double density, vel[3], temp, pert_amp, pert_k, mass;
+  double density, vel[3], temp, pert_amp, pert_k, lo0, mass;
   uint64_t seed_offset;
 };
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+
+// ---- batching form (k_batch.hip) ----
+struct BatchSolveArgs {
+  DevMesh m;
+  const float* val32;       // FP32 stiffness values (ELL layout of m)
+  const int32_t* node_off;  // [nprob+1] node range of each problem
+  double *x, *b, *r64;      // potential, right-hand side, FP64 residual
+  float *rf, *wf, *pf, *sf, *df;  // inner CG vectors
+  int* iters;               // [nprob] inner iterations of the last solve
+  int* converged;           // [nprob]
+  double* relres;           // [nprob] |r| / |b| at the end
+  double tol2, in_tol_min;
+  int max_iter;
+};
+// Every problem's refinement solve, one workgroup of `threads` (256, 512 or 1024) per problem.
+hipError_t launch_batch_ir(const BatchSolveArgs& a, int nprob, int threads, hipStream_t s);
+// Per-problem kinetic energy (exact fixed-point sum at 2^S) and live count over
+// the particle ranges poff[p]..poff[p+1].
+hipError_t launch_batch_stats(const Rec32* r, const int64_t* poff, const double* species, int S, double* ke,
+                              int64_t* count, int nprob, hipStream_t s);
 
 }  // namespace mpic

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libminipic.so")
 OK = 0
 ERR_ARG, ERR_CAPACITY, ERR_NOT_CONVERGED, ERR_HIP, ERR_WALK_CAP, ERR_STATE, ERR_NO_DEVICE = -1, -2, -3, -4, -5, -6, -7
 FORM_FP64, FORM_MIXED = 0, 1
+SOLVER_CG, SOLVER_BICGSTAB = 0, 1
 MAX_SPECIES = 16
 
 # Every extern "C" symbol declared in include/minipic_c.h.
@@ -26,6 +27,9 @@ EXPORTS = [
     "mpic_time_step", "mpic_get_rhs", "mpic_get_phi", "mpic_set_phi", "mpic_get_efield", "mpic_charge_scale",
     "mpic_check_ownership", "mpic_kernel_time", "mpic_mesh_load_vector", "mpic_set_kill_on_wall",
     "mpic_set_species", "mpic_get_config", "mpic_set_rhs",
+    "mpic_batch_create", "mpic_batch_destroy", "mpic_batch_fill", "mpic_batch_tune", "mpic_batch_step",
+    "mpic_batch_graph", "mpic_batch_problem_stats", "mpic_batch_get_phi", "mpic_batch_get_rhs",
+    "mpic_batch_download", "mpic_batch_check_ownership", "mpic_batch_last_error",
 ]
 
 
@@ -42,12 +46,23 @@ class Config(C.Structure):
     _fields_ = [("form", C.c_int), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("kill_on_wall", C.c_int),
                 ("sort_period", C.c_int), ("sort_bucket", C.c_int), ("num_species", C.c_int),
                 ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES),
-                ("solver", C.c_int)]
+                ("solver", C.c_int), ("persistent_solve", C.c_int)]
 
 
 class SoaHost(C.Structure):
     _fields_ = [("x", C.c_void_p), ("x_ref", C.c_void_p), ("v", C.c_void_p), ("elem", C.c_void_p),
                 ("weight", C.c_void_p), ("type", C.c_void_p), ("id", C.c_void_p), ("stride", C.c_int64)]
+
+
+class BatchFill(C.Structure):
+    _fields_ = [("nppe", C.c_int), ("temp", C.c_double), ("density", C.c_double), ("pert_amp", C.c_double),
+                ("drift", C.c_double), ("seed_offset", C.c_uint64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("push_grid", C.c_int), ("solve_threads", C.c_int), ("tuned", C.c_int), ("graph", C.c_int),
+                ("max_inner_iters", C.c_int), ("all_converged", C.c_int), ("num_particles", C.c_int64),
+                ("ms_step", C.c_double)]
 
 
 class MoveStats(C.Structure):
@@ -113,6 +128,18 @@ def lib():
             "mpic_set_species": (i32, [P, i32, P, P]),
             "mpic_get_config": (i32, [P, P]),
             "mpic_set_rhs": (i32, [P, P]),
+            "mpic_batch_create": (i32, [i32, P, P, P]),
+            "mpic_batch_destroy": (None, [P]),
+            "mpic_batch_fill": (i32, [P, P]),
+            "mpic_batch_tune": (i32, [P, d]),
+            "mpic_batch_step": (i32, [P, d, P]),
+            "mpic_batch_graph": (i32, [P, i32]),
+            "mpic_batch_problem_stats": (i32, [P, P, P, P, P, P]),
+            "mpic_batch_get_phi": (i32, [P, i32, P]),
+            "mpic_batch_get_rhs": (i32, [P, i32, P]),
+            "mpic_batch_download": (i32, [P, i32, P, P]),
+            "mpic_batch_check_ownership": (i32, [P, P]),
+            "mpic_batch_last_error": (C.c_char_p, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -347,3 +374,83 @@ class Context:
         ms = C.c_double()
         self._chk(lib().mpic_kernel_time(self.h, which, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """An ensemble of independent problems (the batching form)."""
+
+    def __init__(self, meshes, config=None):
+        self.meshes = list(meshes)
+        self.nprob = len(self.meshes)
+        arr = (C.c_void_p * self.nprob)(*[m.h.value for m in self.meshes])
+        cfg = config if config is not None else default_config()
+        h = C.c_void_p()
+        rc = lib().mpic_batch_create(self.nprob, arr, C.byref(cfg), C.byref(h))
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise MpicError(rc, lib().mpic_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpic_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def fill(self, fills):
+        """fills: one dict per problem with nppe, temp, density, pert_amp, drift, seed_offset."""
+        arr = (BatchFill * self.nprob)()
+        for k, f in enumerate(fills):
+            arr[k] = BatchFill(f["nppe"], f.get("temp", 0.0), f["density"], f.get("pert_amp", 0.0),
+                               f.get("drift", 0.0), f.get("seed_offset", 0))
+        self._chk(lib().mpic_batch_fill(self.h, arr))
+
+    def tune(self, dt):
+        self._chk(lib().mpic_batch_tune(self.h, dt))
+
+    def graph(self, enable):
+        self._chk(lib().mpic_batch_graph(self.h, int(enable)))
+
+    def step(self, dt):
+        st = BatchStats()
+        self._chk(lib().mpic_batch_step(self.h, dt, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+    def problem_stats(self):
+        n = self.nprob
+        ke, rr = np.zeros(n), np.zeros(n)
+        it, cv = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        cnt = np.zeros(n, np.int64)
+        self._chk(lib().mpic_batch_problem_stats(self.h, _p(ke), _p(it), _p(rr), _p(cv), _p(cnt)))
+        return dict(ke=ke, iters=it, relres=rr, converged=cv, count=cnt)
+
+    def phi(self, p):
+        a = np.zeros(self.meshes[p].nn)
+        self._chk(lib().mpic_batch_get_phi(self.h, p, _p(a)))
+        return a
+
+    def rhs(self, p):
+        a = np.zeros(self.meshes[p].nn)
+        self._chk(lib().mpic_batch_get_rhs(self.h, p, _p(a)))
+        return a
+
+    def download(self, p, count):
+        n = count
+        x, xr, v = np.zeros((3, n)), np.zeros((3, n)), np.zeros((3, n))
+        elem, w = np.zeros(n, np.int32), np.zeros(n)
+        typ, ids = np.zeros(n, np.int32), np.zeros(n, np.uint32)
+        s = SoaHost(_p(x), _p(xr), _p(v), _p(elem), _p(w), _p(typ), _p(ids), n)
+        cnt = C.c_int64()
+        self._chk(lib().mpic_batch_download(self.h, p, C.byref(s), C.byref(cnt)))
+        k = cnt.value
+        return dict(x=x[:, :k], xr=xr[:, :k], v=v[:, :k], elem=elem[:k], w=w[:k], type=typ[:k], id=ids[:k])
+
+    def check_ownership(self):
+        n = C.c_int64()
+        self._chk(lib().mpic_batch_check_ownership(self.h, C.byref(n)))
+        return n.value

@@ -810,6 +810,74 @@ int orc_cg(void* h, const double* rhs, double* x, double tol, int max_iter, int*
   return conv ? 0 : 1;
 }
 
+// BiCGStab_solve, CG_solve.hpp:115-206 (the solver PIC::time_step calls,
+// PIC.cpp:251): x is scaled to 0, r = b - A x, r~ = r; each Tpetra
+// update(alpha, A, beta) is this = beta*this + alpha*A; alpha uses rhs.v (r~ = b
+// since x starts at 0).  Stops when r.r < rho0*tol*tol after the alpha step
+// (x += alpha p first) or r.r/rho0 < tol*tol after the omega step; exits early
+// on rho = 0, omega = 0 or t.t = 0 as the reference does.  iters_out = the
+// reference's iter at the exit.  Returns 0 converged, 1 max_iter, 2 early exit.
+int orc_bicgstab(void* h, const double* rhs, double* x, double tol, int max_iter, int* iters_out,
+                 double* relres_out) {
+  Mesh& m = *(Mesh*)h;
+  const int64_t n = m.nn;
+  std::vector<double> r(n), rt(n), p(n), v(n), t(n);
+  const double rho0 = orc_dot(n, rhs, rhs);
+  for (int64_t i = 0; i < n; ++i) x[i] = 0. * x[i];  // x->scale(0.)
+  spmv(m, x, r.data());
+  for (int64_t i = 0; i < n; ++i) r[i] = -1.0 * r[i] + 1.0 * rhs[i];
+  rt = r;
+  double rho = orc_dot(n, rt.data(), r.data()), rho_old = 0, beta = 0, alpha = 0, omega = 0, resid = 0;
+  if (rho == 0) {
+    *iters_out = 0;
+    *relres_out = 0;
+    return 0;
+  }
+  int iter = 0, status = 1;
+  for (; iter < max_iter; ++iter) {
+    if (iter == 0) {
+      p = r;
+    } else {
+      rho = orc_dot(n, rt.data(), r.data());
+      if (rho == 0) {
+        status = 2;
+        break;
+      }
+      beta = rho / rho_old * alpha / omega;
+      for (int64_t i = 0; i < n; ++i) p[i] = 1.0 * p[i] + -omega * v[i];
+      for (int64_t i = 0; i < n; ++i) p[i] = beta * p[i] + 1.0 * r[i];
+    }
+    spmv(m, p.data(), v.data());
+    alpha = orc_dot(n, rhs, v.data());
+    alpha = rho / alpha;
+    for (int64_t i = 0; i < n; ++i) r[i] = 1.0 * r[i] + -alpha * v[i];
+    if ((resid = orc_dot(n, r.data(), r.data())) < rho0 * tol * tol) {
+      for (int64_t i = 0; i < n; ++i) x[i] = 1.0 * x[i] + alpha * p[i];
+      status = 0;
+      break;
+    }
+    spmv(m, r.data(), t.data());
+    const double tdott = orc_dot(n, t.data(), t.data());
+    omega = orc_dot(n, t.data(), r.data()) / tdott;
+    if (omega == 0.0 || tdott == 0) {
+      status = 2;
+      break;
+    }
+    for (int64_t i = 0; i < n; ++i) x[i] = 1.0 * x[i] + alpha * p[i];
+    for (int64_t i = 0; i < n; ++i) x[i] = 1.0 * x[i] + omega * r[i];
+    for (int64_t i = 0; i < n; ++i) r[i] = 1.0 * r[i] + -omega * t[i];
+    rho_old = rho;
+    resid = orc_dot(n, r.data(), r.data());
+    if (resid / rho0 < tol * tol) {
+      status = 0;
+      break;
+    }
+  }
+  *iters_out = iter;
+  *relres_out = std::sqrt(resid / rho0);
+  return status;
+}
+
 // weight_Efield (PIC.cpp:162-189) for every live particle; E is [3][stride].
 void orc_gather(void* h, int64_t n, int64_t stride, const double* xr, const int32_t* elem, const double* phi, double* E) {
   Mesh& m = *(Mesh*)h;

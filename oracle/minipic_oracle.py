@@ -56,6 +56,8 @@ def lib():
         L.orc_spmv.argtypes = [P, P, P]
         L.orc_cg.restype = i32
         L.orc_cg.argtypes = [P, P, P, d, i32, P, P]
+        L.orc_bicgstab.restype = i32
+        L.orc_bicgstab.argtypes = [P, P, P, d, i32, P, P]
         L.orc_gather.argtypes = [P, i64, i64, P, P, P, P]
         L.orc_push.argtypes = [P, i64, i64, P, P, P, P, P, P, P, d, d, i32, P]
         L.orc_ke.restype = d
@@ -200,6 +202,15 @@ def cg(mesh, rhs, phi, tol=1e-10, max_iter=1000):
     rr = np.zeros(1)
     st = lib().orc_cg(mesh.h, ptr(rhs), ptr(phi), tol, max_iter, ptr(it), ptr(rr))
     return st == 0, int(it[0]), float(rr[0])
+
+
+def bicgstab(mesh, rhs, phi, tol=1e-6, max_iter=1000):
+    """BiCGStab_solve from x = 0, in place into phi.  Returns (status, iters, relres):
+    status 0 converged, 1 max_iter, 2 early exit."""
+    it = np.zeros(1, np.int32)
+    rr = np.zeros(1)
+    st = lib().orc_bicgstab(mesh.h, ptr(rhs), ptr(phi), tol, max_iter, ptr(it), ptr(rr))
+    return int(st), int(it[0]), float(rr[0])
 
 
 def gather(mesh, parts, phi):

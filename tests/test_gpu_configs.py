@@ -1,0 +1,66 @@
+"""The irregular-mesh configuration in the precision form, at full size.
+
+128 x 128 x 64 hexes (spacing 1/64) x 6 Kuhn tets = 6,291,456 tets, interior
+nodes jittered by +-20% of the spacing (seeded), 32 electrons per tet as two
+counter-drifting Maxwellians (+-0.2 along x, vth 0.05), uniform neutralising
+background, phi = 0 on the boundary, CG tol 1e-10, dt 0.05, 200 steps.  The
+mesh exceeds the resident grid of the persistent solve, so the per-iteration
+kernels run.  Checks: every solve converges, particle count and ownership are
+kept (reflecting walls, FP64 location), the first solve agrees with the FP64
+form on the same particles, the kinetic energy stays finite and bounded.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DENSITY = 2.0 / 3.0 * math.pi * math.pi
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_c5_jittered_two_stream(mp):
+    say("C5: jittered 128x128x64 x6, 32/tet two streams, precision form, 200 steps")
+    mesh = mp.Mesh(128, 128, 64, 6, hi=(2.0, 2.0, 1.0), jitter=0.2, jitter_seed=5, background=DENSITY)
+    assert mesh.ne == 6291456
+    fills = [((0.2, 0.0, 0.0), 1), ((-0.2, 0.0, 0.0), 2)]
+    ctx = mp.Context(mesh.desc, mp.default_config(form=mp.FORM_MIXED, cg_tol=1e-10))
+    ctx.reserve(mesh.ne * 32)
+    for vel, seed in fills:
+        ctx.fill(16, 0, DENSITY / 2, vel=vel, temp=2 * 0.05 ** 2, seed_offset=seed)
+    n0 = ctx.num_particles()
+    assert n0 == mesh.ne * 32
+    # the first solve on the fill agrees with the FP64 form's on the same particles
+    ctx.weight_charge()
+    ok, it, rr = ctx.solve(1e-10, 1000)
+    phi_mixed = ctx.phi()
+    ref = mp.Context(mesh.desc, mp.default_config(cg_tol=1e-10))
+    ref.reserve(mesh.ne * 32)
+    for vel, seed in fills:
+        ref.fill(16, 0, DENSITY / 2, vel=vel, temp=2 * 0.05 ** 2, seed_offset=seed)
+    ref.weight_charge()
+    ok64, it64, rr64 = ref.solve(1e-10, 1000)
+    phi64 = ref.phi()
+    ref.close()
+    err = np.linalg.norm(phi_mixed - phi64) / np.linalg.norm(phi64)
+    say("  first solve: %d inner iterations (FP64 CG %d), potential L2 difference %.2e" % (it, it64, err))
+    assert ok and ok64 and err < 1e-9
+    ke0 = None
+    for s in range(200):
+        st = ctx.time_step(0.05)
+        assert st["cg_relres"] < 1e-10 and st["killed"] == 0, s
+        if ke0 is None:
+            ke0 = st["kinetic_energy"]
+        if s % 40 == 0:
+            say("  step %d: cg %d iters %.2f ms, push %.2f ms, KE %.6e" % (s, st["cg_iters"], st["ms_solve"],
+                                                                         st["ms_push"], st["kinetic_energy"]))
+    assert ctx.num_particles() == n0
+    assert np.isfinite(st["kinetic_energy"]) and 0.5 * ke0 < st["kinetic_energy"] < 2.0 * ke0
+    assert ctx.check_ownership() == 0
+    ctx.close()

@@ -322,3 +322,32 @@ def test_step_conserves_and_converges(oracle):
     assert p.n == m.ne * 16 and (p.elem[:p.n] >= 0).all()
     assert _contained(m, p) > -1e-10
     assert abs(ke[-1] - ke[0]) / ke[0] < 0.05
+
+
+@pytest.mark.parametrize("split", [5, 6])
+def test_bicgstab_against_scipy(oracle, split):
+    """BiCGStab_solve (CG_solve.hpp:115-206): the solution of K x = b matches a
+    direct solve, the initial x is discarded (x is scaled to 0), and the
+    reported relative residual is the recurrence's r.r / b.b, which stays
+    within a small factor of the true residual."""
+    import scipy.sparse.linalg as sla
+    m = oracle.Mesh(5, 5, 5, split, lo=(-1, -1, -1), hi=(1, 1, 1))
+    A = m.csr_scipy()
+    rng = np.random.default_rng(split)
+    b = rng.standard_normal(m.nn)
+    diag_only = np.diff(A.indptr) == 1
+    b[diag_only] = 0.0
+    xs = sla.spsolve(A.tocsc(), b)
+    x1 = rng.standard_normal(m.nn)
+    x2 = np.zeros(m.nn)
+    s1, it1, r1 = oracle.bicgstab(m, b, x1, tol=1e-12)
+    s2, it2, r2 = oracle.bicgstab(m, b, x2, tol=1e-12)
+    assert s1 == 0 and s2 == 0 and it1 == it2 and r1 == r2
+    assert np.array_equal(x1, x2)
+    assert np.abs(x1 - xs).max() <= 1e-9 * np.abs(xs).max()
+    true = np.linalg.norm(A @ x1 - b) / np.linalg.norm(b)
+    assert r1 < 1e-12 and true < 1e-10
+    # a loose tolerance stops early with the reference's test r.r < b.b tol^2
+    x3 = np.zeros(m.nn)
+    s3, it3, r3 = oracle.bicgstab(m, b, x3, tol=1e-3)
+    assert s3 == 0 and it3 < it1 and r3 < 1e-3
