@@ -923,8 +923,8 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     c->perm_pending = false;
   }
   PushPartial r{};
-  if (c->n > 0) {
-    if ((rc = finish_partials(c, partials_for(c->n), &r))) return rc;
+  if (a.n > 0) {
+    if ((rc = finish_partials(c, partials_for(a.n), &r))) return rc;  // the push's grid
   }
   tstop(c, T_PUSH);
   c->acc_deposited = true;

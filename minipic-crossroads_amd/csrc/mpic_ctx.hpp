@@ -43,7 +43,7 @@ struct mpic_ctx {
   // into the other one).  The live state is P[cur] (FP64) or P32[cur32]
   // (precision form, state32); n counts its positions.
   int64_t cap = 0, n = 0;
-  int32_t* perm = nullptr;  // [cap] sort order of the live set from the last ranking
+  int32_t* perm = nullptr;  // [cap] sort order of the live set from the last ranking (slot -> position)
   bool perm_pending = false;
   bool perm_by_bucket = false;  // perm orders by sort bucket, not by element
   int64_t perm_n = 0;       // live particles listed in perm

@@ -7,7 +7,10 @@ background, phi = 0 on the boundary, CG tol 1e-10, dt 0.05, 200 steps.  The
 mesh exceeds the resident grid of the persistent solve, so the per-iteration
 kernels run.  Checks: every solve converges, particle count and ownership are
 kept (reflecting walls, FP64 location), the first solve agrees with the FP64
-form on the same particles, the kinetic energy stays finite and bounded.
+form on the same particles, the kinetic energy stays finite and bounded.  The
+fill is neutral element by element (equal-weight electrons against the uniform
+background), so the net charge is a tiny difference of large terms: the
+potential's error is measured against the potential of the background alone.
 """
 import math
 import sys
@@ -48,8 +51,15 @@ def test_c5_jittered_two_stream(mp):
     ok64, it64, rr64 = ref.solve(1e-10, 1000)
     phi64 = ref.phi()
     ref.close()
-    err = np.linalg.norm(phi_mixed - phi64) / np.linalg.norm(phi64)
-    say("  first solve: %d inner iterations (FP64 CG %d), potential L2 difference %.2e" % (it, it64, err))
+    bgctx = mp.Context(mesh.desc, mp.default_config(cg_tol=1e-10))
+    bgctx.reserve(1)
+    bgctx.weight_charge()  # no particles: the background's load vector
+    assert bgctx.solve(1e-10, 1000)[0]
+    scale = np.abs(bgctx.phi()).max()
+    bgctx.close()
+    err = np.abs(phi_mixed - phi64).max() / scale
+    say("  first solve: %d inner iterations (FP64 CG %d), max potential difference %.2e of the background's"
+        % (it, it64, err))
     assert ok and ok64 and err < 1e-9
     ke0 = None
     for s in range(200):
