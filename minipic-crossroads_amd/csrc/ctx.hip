@@ -538,7 +538,8 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
   if ((rc = dalloc(c, &c->m.ephi, (size_t)ne * 4))) return bail(rc);
   if ((rc = dalloc(c, &c->phi, nn)) || (rc = dalloc(c, &c->rhs, nn)) || (rc = dalloc(c, &c->acc, (size_t)ne * 16)) ||
       (rc = dalloc(c, &c->cg.r, nn)) || (rc = dalloc(c, &c->cg.p0, nn)) || (rc = dalloc(c, &c->cg.p1, nn)) ||
-      (rc = dalloc(c, &c->cg.q, nn)) || (rc = dalloc(c, &c->cg.part_bb, nchunks)) ||
+      (rc = dalloc(c, &c->cg.q, nn)) || (rc = dalloc(c, &c->cg.pr[0], nn)) || (rc = dalloc(c, &c->cg.pr[1], nn)) ||
+      (rc = dalloc(c, &c->cg.part_bb, nchunks)) ||
       (rc = dalloc(c, &c->cg.part_rr, nchunks)) || (rc = dalloc(c, &c->cg.part_pq, nchunks)) ||
       (rc = dalloc(c, &c->cg.st, 1)) || (rc = dalloc(c, &c->sb.count, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.start, (size_t)ne + 2)) || (rc = dalloc(c, &c->sb.cursor, (size_t)ne + 2)) ||

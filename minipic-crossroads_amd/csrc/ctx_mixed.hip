@@ -60,6 +60,7 @@ int mixed_prepare(mpic_ctx* c) {
         (rc = dalloc(c, &c->ir.rv[0], nn)) || (rc = dalloc(c, &c->ir.rv[1], nn)) ||
         (rc = dalloc(c, &c->ir.wv[0], nn)) || (rc = dalloc(c, &c->ir.wv[1], nn)) ||
         (rc = dalloc(c, &c->ir.sv[0], nn)) || (rc = dalloc(c, &c->ir.sv[1], nn)) ||
+        (rc = dalloc(c, &c->ir.q[0], nn)) || (rc = dalloc(c, &c->ir.q[1], nn)) ||
         (rc = dalloc(c, &c->ir.part_a, (size_t)2 * c->cg.nchunks)) ||
         (rc = dalloc(c, &c->ir.part_b, (size_t)4 * c->cg.nchunks)) || (rc = dalloc(c, &c->ir.st, 1)) ||
         (rc = dalloc(c, &c->bad, 1)))

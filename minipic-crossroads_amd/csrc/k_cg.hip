@@ -67,19 +67,23 @@ __device__ __forceinline__ double ell_row(const DevMesh& m, int row, F xval) {
   return acc;
 }
 
-// One ELL row staged in registers (columns, values and both gathered vectors),
+// The iteration state lives in one 16-B word per row, (p_{i-1}, r_i), so the
+// two vectors a row of A p needs at a column come with one load; the words
+// alternate between two buffers by iteration parity.
+// One ELL row staged in registers (columns, values and the gathered words),
 // loaded before anything waits on a reduction.  The sums below keep the
 // sequential CSR order of ell_row exactly; rows longer than ROW_REGS read their
-// tail directly.
-constexpr int ROW_REGS = 16;
+// tail directly.  Ten staged entries keep k_cg_k1 within 96 VGPRs, i.e. five
+// waves per SIMD: 1280 resident workgroups, so a 65^3-node mesh (1073 chunks)
+// runs in one round instead of two.
+constexpr int ROW_REGS = 10;
 
 struct Row64 {
   int len;
   double v[ROW_REGS], a[ROW_REGS], b[ROW_REGS];
 };
 
-__device__ __forceinline__ void stage_row(const DevMesh& m, int row, const double* __restrict__ va,
-                                          const double* __restrict__ vb, Row64& R) {
+__device__ __forceinline__ void stage_row(const DevMesh& m, int row, const double2* __restrict__ pr, Row64& R) {
   const int slice = row >> 6, lane = row & 63;
   R.len = m.ell_len[slice];
   const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
@@ -90,14 +94,15 @@ __device__ __forceinline__ void stage_row(const DevMesh& m, int row, const doubl
 #pragma unroll
   for (int k = 0; k < ROW_REGS; ++k) {
     R.v[k] = k < R.len ? vp[k * 64] : 0.0;
-    R.a[k] = va ? va[c[k]] : 0.0;
-    R.b[k] = vb[c[k]];
+    const double2 t = pr[c[k]];
+    R.a[k] = t.x;
+    R.b[k] = t.y;
   }
 }
 
-// acc = 0; acc += v_k * x_k in CSR order, x_k = b_k (first iteration) or beta a_k + b_k.
+// acc = 0; acc += v_k * x_k in CSR order, x_k = r_k (first iteration) or beta p_k + r_k.
 __device__ __forceinline__ double row_apply(const DevMesh& m, int row, const Row64& R, bool first, double beta,
-                                            const double* __restrict__ va, const double* __restrict__ vb) {
+                                            const double2* __restrict__ pr) {
   double acc = 0;
 #pragma unroll
   for (int k = 0; k < ROW_REGS; ++k)
@@ -107,8 +112,8 @@ __device__ __forceinline__ double row_apply(const DevMesh& m, int row, const Row
     const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
     const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
     for (int k = ROW_REGS; k < R.len; ++k) {
-      const int cc = cp[k * 64];
-      acc += vp[k * 64] * (first ? vb[cc] : beta * va[cc] + vb[cc]);
+      const double2 t = pr[cp[k * 64]];
+      acc += vp[k * 64] * (first ? t.y : beta * t.x + t.y);
     }
   }
   return acc;
@@ -116,53 +121,59 @@ __device__ __forceinline__ double row_apply(const DevMesh& m, int row, const Row
 
 struct CGArgs {
   DevMesh m;
-  double *x, *b, *r, *pold, *pnew, *q;
+  double *x, *b, *q;
+  const double2* pr_cur;  // (p_{i-1}, r_i) (read)
+  double2* pr_nxt;        // (p_i, r_{i+1}) (written: p by k1, both by k2)
   double *part_bb, *part_rr, *part_pq;
   double* rho_hist;  // [max_iter+1] rho of each iteration
   CGState* st;
   int it;
+  int nchunks;  // row chunks of BLOCK rows; partials are indexed by chunk
 };
 
 // r = b - A x; partials of b.b and r.r (CG_solve.hpp:58-66).
 __global__ __launch_bounds__(BLOCK) void k_cg_init(CGArgs a) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
+  if (chunk < 0) return;
+  const int i = chunk * BLOCK + threadIdx.x;
   double tb = 0, tr = 0;
   if (i < a.m.nn) {
     const double bi = a.b[i];
     const double* __restrict__ x = a.x;
     const double ax = ell_row(a.m, i, [&](int c) { return x[c]; });
     const double ri = bi - ax;
-    a.r[i] = ri;
+    a.pr_nxt[i] = make_double2(0.0, ri);  // the state of iteration 0 (p_{-1} unused)
     tb += bi * bi;
     tr += ri * ri;
   }
   const double sb = block_tree(tb);
   const double sr = block_tree(tr);
   if (threadIdx.x == 0) {
-    a.part_bb[blockIdx.x] = sb;
-    a.part_rr[blockIdx.x] = sr;
+    a.part_bb[chunk] = sb;
+    a.part_rr[chunk] = sr;
   }
 }
 
 // Convergence test (CG_solve.hpp:71-72,103-107; max_iter :73,113), then
 // p = beta p_old + r fused into q = A p; partials of p.q (CG_solve.hpp:77-87).
 // The row's operands are loaded before the reduction of the previous partials.
-__global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
+__global__ __launch_bounds__(BLOCK, 5) void k_cg_k1(CGArgs a) {
   CGState* st = a.st;
-  if (st->done) return;
+  const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
+  if (chunk < 0 || st->done) return;
   const int it = a.it;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const int i = chunk * BLOCK + threadIdx.x;
   const bool row = i < a.m.nn;
-  const double* __restrict__ po = it == 0 ? nullptr : a.pold;
   Row64 R;
   double poi = 0, ri = 0;
   if (row) {
-    stage_row(a.m, i, po, a.r, R);
-    poi = po ? po[i] : 0.0;
-    ri = a.r[i];
+    stage_row(a.m, i, a.pr_cur, R);
+    const double2 own = a.pr_cur[i];
+    poi = own.x;
+    ri = own.y;
   }
-  const double rho = reduce_partials(a.part_rr, gridDim.x);
-  const double rho0 = it == 0 ? reduce_partials(a.part_bb, gridDim.x) : st->rho0;
+  const double rho = reduce_partials(a.part_rr, a.nchunks);
+  const double rho0 = it == 0 ? reduce_partials(a.part_bb, a.nchunks) : st->rho0;
   bool stop = false, conv = false;
   if (it == 0) {
     if (rho == 0) stop = conv = true;
@@ -170,7 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
     stop = conv = true;
   }
   if (!stop && it >= st->max_iter) stop = true;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (chunk == 0 && threadIdx.x == 0) {
     st->rho = rho;
     if (it == 0) st->rho0 = rho0;
     a.rho_hist[it] = rho;
@@ -186,43 +197,44 @@ __global__ __launch_bounds__(BLOCK) void k_cg_k1(CGArgs a) {
     double pi, qi;
     if (it == 0) {
       pi = ri;
-      qi = row_apply(a.m, i, R, true, 0.0, po, a.r);
+      qi = row_apply(a.m, i, R, true, 0.0, a.pr_cur);
     } else {
       const double beta = rho / a.rho_hist[it - 1];
       pi = beta * poi + ri;
-      qi = row_apply(a.m, i, R, false, beta, po, a.r);
+      qi = row_apply(a.m, i, R, false, beta, a.pr_cur);
     }
-    a.pnew[i] = pi;
+    a.pr_nxt[i].x = pi;
     a.q[i] = qi;
     t += pi * qi;
   }
   const double s = block_tree(t);
-  if (threadIdx.x == 0) a.part_pq[blockIdx.x] = s;
+  if (threadIdx.x == 0) a.part_pq[chunk] = s;
 }
 
 // alpha = rho / p.q; x += alpha p; r += (-alpha) q; partials of r.r (CG_solve.hpp:87-100).
 __global__ __launch_bounds__(BLOCK) void k_cg_k2(CGArgs a) {
-  if (a.st->done) return;
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
+  if (chunk < 0 || a.st->done) return;
+  const int i = chunk * BLOCK + threadIdx.x;
   const bool row = i < a.m.nn;
   double xi = 0, pi = 0, ri = 0, qi = 0;
   if (row) {
     xi = a.x[i];
-    pi = a.pnew[i];
-    ri = a.r[i];
+    pi = a.pr_nxt[i].x;
+    ri = a.pr_cur[i].y;
     qi = a.q[i];
   }
-  double alpha = reduce_partials(a.part_pq, gridDim.x);
+  double alpha = reduce_partials(a.part_pq, a.nchunks);
   alpha = a.rho_hist[a.it] / alpha;
   double t = 0;
   if (row) {
     a.x[i] = xi + alpha * pi;
     const double rn = ri + -alpha * qi;
-    a.r[i] = rn;
+    a.pr_nxt[i] = make_double2(pi, rn);
     t += rn * rn;
   }
   const double s = block_tree(t);
-  if (threadIdx.x == 0) a.part_rr[blockIdx.x] = s;
+  if (threadIdx.x == 0) a.part_rr[chunk] = s;
 }
 
 __global__ void k_cg_setup(CGState* st, double tol2, int max_iter) {
@@ -237,9 +249,8 @@ CGArgs make_args(const DevMesh& m, CGBuffers& c, int it) {
   a.m = m;
   a.x = c.x;
   a.b = c.b;
-  a.r = c.r;
-  a.pold = (it & 1) ? c.p1 : c.p0;
-  a.pnew = (it & 1) ? c.p0 : c.p1;
+  a.pr_cur = c.pr[it & 1];
+  a.pr_nxt = c.pr[(it & 1) ^ 1];
   a.q = c.q;
   a.part_bb = c.part_bb;
   a.part_rr = c.part_rr;
@@ -247,6 +258,7 @@ CGArgs make_args(const DevMesh& m, CGBuffers& c, int it) {
   a.rho_hist = c.rho_hist;
   a.st = c.st;
   a.it = it;
+  a.nchunks = c.nchunks;
   return a;
 }
 
@@ -254,7 +266,8 @@ CGArgs make_args(const DevMesh& m, CGBuffers& c, int it) {
 
 hipError_t launch_cg_init(const DevMesh& m, CGBuffers& c, double tol, int max_iter, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(1), 0, s, c.st, tol * tol, max_iter);
-  hipLaunchKernelGGL(k_cg_init, dim3(c.nchunks), dim3(BLOCK), 0, s, make_args(m, c, 0));
+  // k_cg_init writes the parity-0 words (the arguments of iteration -1).
+  hipLaunchKernelGGL(k_cg_init, dim3(xcd_grid(c.nchunks)), dim3(BLOCK), 0, s, make_args(m, c, -1));
   return hipGetLastError();
 }
 
@@ -265,8 +278,9 @@ hipError_t launch_cg_setup(CGBuffers& c, double tol, int max_iter, hipStream_t s
 
 hipError_t launch_cg_iter(const DevMesh& m, CGBuffers& c, int it, int max_iter, hipStream_t s) {
   const CGArgs a = make_args(m, c, it);
-  hipLaunchKernelGGL(k_cg_k1, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
-  if (it < max_iter) hipLaunchKernelGGL(k_cg_k2, dim3(c.nchunks), dim3(BLOCK), 0, s, a);
+  const dim3 grid(xcd_grid(c.nchunks));
+  hipLaunchKernelGGL(k_cg_k1, grid, dim3(BLOCK), 0, s, a);
+  if (it < max_iter) hipLaunchKernelGGL(k_cg_k2, grid, dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

@@ -5,6 +5,19 @@
 
 namespace mpic {
 
+// The MI355X dispatches workgroups to its 8 XCDs round-robin (workgroup b runs
+// on XCD b mod 8), so a fixed workgroup -> chunk map keeps every chunk on one
+// XCD from launch to launch.  This map gives each XCD one contiguous eighth of
+// the n row chunks: the matrix rows an XCD reads and nearly all the vector
+// entries its rows gather then stay in that XCD's L2 across the iterations of a
+// solve (with the plain map each XCD gathers from the whole vector).  Returns -1
+// for the padding workgroups of a grid of xcd_grid(n).
+__device__ __forceinline__ int xcd_chunk(int b, int n) {
+  const int per = (n + NUM_XCD - 1) / NUM_XCD;
+  const int c = (b % NUM_XCD) * per + b / NUM_XCD;
+  return c < n ? c : -1;
+}
+
 // 12 / 16 consecutive doubles of element e (records are 96 / 128 B, 16-B aligned).
 __device__ __forceinline__ void load12(const double* __restrict__ base, int e, double o[12]) {
   const double2* q = reinterpret_cast<const double2*>(base + (size_t)e * 12);

@@ -18,6 +18,11 @@ constexpr int ELL_SLICE = 64;      // rows per ELL slice = one wavefront
 constexpr int MAX_PARTIALS = 8192; // workgroups of the grid-stride particle kernels (upper bound)
 constexpr int DEFAULT_GRID = 4096; // their default number of workgroups
 constexpr int IR_MAX_OUTER = 40;   // refinement passes of the precision-form solve
+constexpr int NUM_XCD = 8;         // accelerator dies of the MI355X, each with its own L2
+
+// Row-chunk kernels launch xcd_grid(n) workgroups for n chunks of BLOCK rows and
+// take their chunk from xcd_chunk (mpic_device.hpp).
+inline int xcd_grid(int n) { return (n + NUM_XCD - 1) / NUM_XCD * NUM_XCD; }
 
 // 128-bit two's-complement fixed-point value (lo word first).
 struct fx128 {
@@ -233,7 +238,8 @@ int partials_for(int64_t n);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 
 struct CGBuffers {
-  double *x, *b, *r, *p0, *p1, *q;
+  double *x, *b, *r, *p0, *p1, *q;  // r, p0, p1: the persistent solve's (and BiCGStab's scratch)
+  double2* pr[2];                   // (p_{i-1}, r_i) words by iteration parity (per-iteration kernels)
   double *part_bb, *part_rr, *part_pq;
   double* rho_hist;  // [max_iter+1]
   CGState* st;
@@ -318,8 +324,9 @@ struct IRState {
 };
 struct IRBuffers {
   double *x, *b, *r;              // FP64 solution, right-hand side, residual
-  float *d, *p;                   // FP32 inner correction and direction
-  float *rv[2], *wv[2], *sv[2];   // FP32 residual r, w = A r, s = A p; by iteration parity
+  float *d, *p;                   // FP32 inner correction; direction p (persistent solve)
+  float *rv[2], *wv[2], *sv[2];   // FP32 r, w = A r, s = A p by iteration parity (persistent solve)
+  float4* q[2];                   // the same state as (r, w, s, p) words by parity (per-iteration kernels)
   double* part_a;                 // [2*nchunks]: b.b, r.r of the outer residual
   double* part_b;                 // [4*nchunks]: r.r, w.r of the inner residual, by parity
   double *rho_hist, *alpha_hist;  // [max_iter+1]
