@@ -146,7 +146,7 @@ PushArgs32 push_args(mpic_batch* b, double dt, unsigned long long* acc, int32_t*
 }
 
 int push_grid_used(mpic_batch* b) {
-  return b->push_grid > 0 ? std::min(b->push_grid, MAX_PARTIALS) : (int)partials_for(b->c->n);
+  return b->push_grid > 0 ? std::min(b->push_grid, MAX_PARTIALS) : push32_grid(b->c->n);
 }
 
 // One step of every problem as a fixed sequence of launches (no host

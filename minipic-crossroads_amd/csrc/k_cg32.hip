@@ -3,9 +3,14 @@
 // |r|^2 < tol^2 |b|^2 (CG_solve.hpp:71-72,103-107) on the FP64 residual.
 //
 // Outer pass (FP64): r = b - A x; stop when r.r < tol^2 b.b; otherwise an inner
-// FP32 solve of A d = r/|r| from d = 0 to the relative residual
+// FP32 pass on A d = r/|r| from d = 0 to the relative residual
 // max(in_tol_min, 0.3 tol |b|/|r|), then x += |r| d.  The FP32 matrix holds the
-// FP64 stiffness values rounded to FP32 in the same sliced ELL layout.
+// FP64 stiffness values rounded to FP32 in the same sliced ELL layout.  Passes
+// are reliable updates of one CG run (Sleijpen and van der Vorst; van der
+// Vorst and Ye): the true FP64 residual replaces the recursive FP32 one, and
+// the search direction p and s = A p carry over (rescaled to the new pass's
+// units), so the iteration count stays near the FP64 CG's instead of paying
+// for a restart at every pass.
 //
 // The inner solve is the single-reduction form of CG (Chronopoulos and Gear):
 // with w = A r and s = A p carried as vectors, both dot products of an
@@ -100,6 +105,7 @@ struct IRArgs {
   float4* q_nxt;               // (r_{i+1}, w_{i+1}, s_i, p_i) (written)
   double *part_bb, *part_rr;   // outer partials
   double *part_cur, *part_nxt; // inner partials [2*nchunks]: r.r then w.r
+  double* part_e;              // [2*nchunks]: r.s', p.s' at the start of a continued pass
   double *rho_hist, *alpha_hist;
   IRState* st;
   int it;
@@ -134,19 +140,29 @@ __global__ __launch_bounds__(BLOCK) void k_ir_resid(IRArgs a) {
   }
 }
 
-// Outer convergence test; otherwise starts the inner solve: r_0 = r/|r| (FP32),
-// w_0 = A r_0 (each column's r_0 formed from the FP64 residual), d = 0, and the
-// partials of r_0.r_0 and w_0.r_0.
+// Outer convergence test; otherwise starts an inner pass on r' = r/|r| (FP32)
+// with w' = A r' (each column's r' formed from the FP64 residual), d = 0, and
+// the partials of r'.r' and w'.r'.  When the previous pass stopped at its
+// tolerance, the pass continues its direction (reliable update of the
+// residual): p' = c p and s' = c s with c = |r_prev| / |r| (the previous
+// pass's units to this one's), the previous last r.r rescaled by c^2 for beta,
+// and the partials of r'.s' and p'.s' for the exact p.Ap of the first step.
+// Otherwise p' = s' = 0 (a fresh start).  Reads only state fields that no
+// workgroup of this launch writes.
 __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min) {
   IRState* st = a.st;
   const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
   if (chunk < 0 || st->done) return;
+  const int outer = st->outer;
+  const int cont = outer > 0 && st->cont_ok;
+  const double scale_prev = st->scale_prev, rho_last = st->rho_last, tol2 = st->tol2;
+  const int in_last = st->in_last;
   const double bb = reduce(a.part_bb, a.nchunks);
   const double rr = reduce(a.part_rr, a.nchunks);
-  const int outer = st->outer;
-  const bool conv = rr == 0 || rr < st->tol2 * bb;
+  const bool conv = rr == 0 || rr < tol2 * bb;
   const bool stop = conv || outer >= IR_MAX_OUTER;
   const double scale = sqrt(rr);
+  const double cs = cont ? scale_prev / scale : 0.0;
   if (chunk == 0 && threadIdx.x == 0) {
     st->bb = bb;
     st->rr = rr;
@@ -154,18 +170,17 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
       st->done = 1;
       st->converged = conv ? 1 : 0;
     } else {
-      st->outer = outer + 1;
-      st->in_done = 0;
-      st->in_iters = 0;
       st->scale = scale;
-      const double want = 0.3 * sqrt(st->tol2 * bb / rr);
+      st->cont = cont;
+      st->rho_c = cs * cs * rho_last;
+      const double want = 0.3 * sqrt(tol2 * bb / rr);
       const double eps = want > in_tol_min ? want : in_tol_min;
       st->in_tol2 = eps * eps;
     }
   }
   if (stop) return;
   const int i = chunk * BLOCK + threadIdx.x;
-  double tg = 0, td = 0;
+  double tg = 0, td = 0, te = 0, tf = 0;
   if (i < a.m.nn) {
     const double inv = 1.0 / scale;
     const int slice = i >> 6, lane = i & 63;
@@ -175,7 +190,18 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
     float w = 0;
     for (int k = 0; k < len; ++k) w = fmaf(vp[k * 64], (float)(a.r[cp[k * 64]] * inv), w);
     const float ri = (float)(a.r[i] * inv);
-    a.q_nxt[i] = make_float4(ri, w, 0.0f, 0.0f);  // the state of iteration 0
+    float sc = 0.0f, pc = 0.0f;
+    if (cont) {
+      // the previous pass stopped at iteration in_iters: its last (s, p) are in
+      // the words of that parity (q_cur = parity 1, q_nxt = parity 0 here)
+      const float4 last = (in_last & 1) ? a.q_cur[i] : a.q_nxt[i];
+      const float cf = (float)cs;
+      sc = cf * last.z;
+      pc = cf * last.w;
+      te = (double)ri * (double)sc;
+      tf = (double)pc * (double)sc;
+    }
+    a.q_nxt[i] = make_float4(ri, w, sc, pc);  // the state of iteration 0
     a.d[i] = 0.0f;
     tg = (double)ri * (double)ri;
     td = (double)w * (double)ri;
@@ -186,12 +212,23 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
     a.part_nxt[chunk] = sg;
     a.part_nxt[a.nchunks + chunk] = sd;
   }
+  if (cont) {
+    const double se = block_sum(te);
+    const double sf = block_sum(tf);
+    if (threadIdx.x == 0) {
+      a.part_e[chunk] = se;
+      a.part_e[a.nchunks + chunk] = sf;
+    }
+  }
 }
 
 // One inner iteration i (Chronopoulos-Gear CG):
 //   gamma = r_i.r_i, delta = w_i.r_i (partials of the previous launch);
-//   beta = gamma / gamma_{i-1}, alpha = gamma / (delta - beta gamma / alpha_{i-1})
-//   (i = 0: beta = 0, alpha = gamma / delta);
+//   beta = gamma / gamma_{i-1}, alpha = gamma / (delta - beta gamma / alpha_{i-1});
+//   at i = 0 of a fresh pass beta = 0, alpha = gamma / delta; at i = 0 of a
+//   continued pass beta = gamma / rho_c and p.Ap = delta + 2 beta r.s' +
+//   beta^2 p'.s' exactly (the previous pass's direction is not A-conjugate to
+//   the replaced residual, so the short form of p.Ap does not hold);
 //   p_i = r_i + beta p_{i-1}; s_i = w_i + beta s_{i-1}; d += alpha p_i;
 //   r_{i+1} = r_i - alpha s_i; w_{i+1} = A r_{i+1}; partials of r_{i+1}.r_{i+1}
 //   and w_{i+1}.r_{i+1}.
@@ -206,7 +243,9 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   const int done = st->done | st->in_done;
   const double rho0_prev = st->in_rho0, tol2 = st->in_tol2;
   const int max_iter = st->max_iter;
-  const double rho_prev = it > 0 ? a.rho_hist[it - 1] : 1.0;
+  const bool cont0 = it == 0 && st->cont;
+  const bool carry = it > 0 || cont0;  // beta != 0: the update uses p and s
+  const double rho_prev = it > 0 ? a.rho_hist[it - 1] : st->rho_c;
   const double alpha_prev = it > 0 ? a.alpha_hist[it - 1] : 1.0;
   const int i = chunk * BLOCK + threadIdx.x;
   const bool row = i < a.m.nn;
@@ -231,6 +270,11 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   if (it > 0) {
     beta = gamma / rho_prev;
     denom = delta - beta * gamma / alpha_prev;
+  } else if (cont0) {
+    const double rs = reduce(a.part_e, a.nchunks);
+    const double ps = reduce(a.part_e + a.nchunks, a.nchunks);
+    beta = gamma / rho_prev;
+    denom = fma(beta * beta, ps, fma(2.0 * beta, rs, delta));  // same rounding in every build
   }
   const bool conv = gamma == 0 || (it > 0 && gamma < tol2 * rho0);
   const bool stop = conv || it >= max_iter || !(denom > 0);
@@ -242,6 +286,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
     a.alpha_hist[it] = alpha;
     if (stop) {
       st->in_done = 1;
+      st->in_ok = conv && it > 0 ? 1 : 0;
       st->in_iters = it;
       st->total_inner += it;
     }
@@ -251,14 +296,14 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   double tg = 0, td = 0;
   if (row) {
     const float ri = own.x, wi = own.y;
-    const float p = it > 0 ? fmaf(bf, own.w, ri) : ri;
-    const float s = it > 0 ? fmaf(bf, own.z, wi) : wi;
+    const float p = carry ? fmaf(bf, own.w, ri) : ri;
+    const float s = carry ? fmaf(bf, own.z, wi) : wi;
     const float rn = fmaf(-af, s, ri);
     a.d[i] = fmaf(af, p, di);
     float w = 0;
 #pragma unroll
     for (int k = 0; k < ROW_REGS; ++k) {
-      const float sk = it > 0 ? fmaf(bf, R.g[k].z, R.g[k].y) : R.g[k].y;
+      const float sk = carry ? fmaf(bf, R.g[k].z, R.g[k].y) : R.g[k].y;
       w = fmaf(R.v[k], fmaf(-af, sk, R.g[k].x), w);
     }
     if (R.len > ROW_REGS) {
@@ -267,7 +312,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
       const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
       for (int k = ROW_REGS; k < R.len; ++k) {
         const float4 g = a.q_cur[cp[k * 64]];
-        const float sk = it > 0 ? fmaf(bf, g.z, g.y) : g.y;
+        const float sk = carry ? fmaf(bf, g.z, g.y) : g.y;
         w = fmaf(vp[k * 64], fmaf(-af, sk, g.x), w);
       }
     }
@@ -283,12 +328,26 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   }
 }
 
-// x += |r| d in FP64.
+// x += |r| d in FP64; workgroup 0 then records what the next pass's start reads
+// (the pass count, this pass's scale, its last r.r, whether it may be
+// continued) and clears the inner state.
 __global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a) {
   const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
-  if (chunk < 0 || a.st->done) return;
+  IRState* st = a.st;
+  if (chunk < 0 || st->done) return;
   const int i = chunk * BLOCK + threadIdx.x;
-  if (i < a.m.nn) a.x[i] = fma(a.st->scale, (double)a.d[i], a.x[i]);
+  if (i < a.m.nn) a.x[i] = fma(st->scale, (double)a.d[i], a.x[i]);
+  if (chunk == 0 && threadIdx.x == 0) {
+    const int last = st->in_iters;
+    st->outer = st->outer + 1;
+    st->cont_ok = st->in_ok && last > 0 ? 1 : 0;
+    st->scale_prev = st->scale;
+    st->rho_last = last > 0 ? a.rho_hist[last - 1] : 0.0;
+    st->in_last = last;
+    st->in_done = 0;
+    st->in_ok = 0;
+    st->in_iters = 0;
+  }
 }
 
 // Arguments of inner iteration it (k_ir_start fills the parity-0 buffers: it = -1).
@@ -307,6 +366,7 @@ IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
   a.part_rr = c.part_a + c.nchunks;
   a.part_cur = c.part_b + (size_t)2 * c.nchunks * cur;
   a.part_nxt = c.part_b + (size_t)2 * c.nchunks * nxt;
+  a.part_e = c.part_e;
   a.rho_hist = c.rho_hist;
   a.alpha_hist = c.alpha_hist;
   a.st = c.st;
@@ -319,8 +379,8 @@ IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
 
 hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bool first, double tol, int max_iter,
                            double in_tol_min, hipStream_t s) {
-  if (!c.x || !c.b || !c.r || !c.d || !c.q[0] || !c.q[1] || !c.part_a || !c.part_b || !c.rho_hist || !c.alpha_hist ||
-      !c.st || !m32.ell_val || c.nchunks <= 0)
+  if (!c.x || !c.b || !c.r || !c.d || !c.q[0] || !c.q[1] || !c.part_a || !c.part_b || !c.part_e || !c.rho_hist ||
+      !c.alpha_hist || !c.st || !m32.ell_val || c.nchunks <= 0)
     return hipErrorInvalidValue;
   if (first) hipLaunchKernelGGL(k_ir_setup, dim3(1), dim3(1), 0, s, c.st, tol * tol, max_iter);
   const IRArgs a = make_args(m, m32, c, -1);

@@ -234,6 +234,7 @@ struct PIRArgs {
   float *rv0, *rv1, *wv0, *wv1, *sv0, *sv1;
   double* part_a;  // [2G]: b.b, r.r of the outer residual
   double* part_b;  // [2][2G]: r.r, w.r of the inner residual by iteration parity
+  double* part_e;  // [2G]: r.s', p.s' at the start of a continued pass
   IRState* st;
   GridSync sync;
   double tol2, in_tol_min;
@@ -241,7 +242,8 @@ struct PIRArgs {
 };
 
 // The refinement solve of k_cg32.hip in one launch: per pass the FP64
-// residual, the start of the inner solve on r/|r|, the Chronopoulos-Gear
+// residual, the start of the inner pass on r/|r| (continuing the previous
+// pass's direction when it stopped at its tolerance), the Chronopoulos-Gear
 // iterations and x += |r| d, with the arithmetic of k_ir_resid, k_ir_start,
 // k_ir_cg and k_ir_update.
 __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
@@ -253,6 +255,9 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
   double xi = has ? a.x[row] : 0.0;
   unsigned ep = 0;
   int total_inner = 0;
+  float si = 0, pi = 0;  // the last pass's direction and A times it (this row)
+  bool cont_ok = false;
+  double scale_prev = 0, rho_last = 0;
   for (int outer = 0;; ++outer) {
     // r = b - A x (FP64); partials of b.b and r.r
     double tb = 0, tr = 0;
@@ -293,8 +298,11 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       }
     }
     if (stop) break;
-    float ri = 0, wi = 0, si = 0, pi = 0, di = 0;
-    double tg = 0, td = 0;
+    const bool cont = outer > 0 && cont_ok;
+    const double cs = cont ? scale_prev / scale : 0.0;
+    const double rho_c = cs * cs * rho_last;
+    float ri = 0, wi = 0, di = 0;
+    double tg = 0, td = 0, te = 0, tf = 0;
     if (has) {
       const double inv = 1.0 / scale;
       const int len = a.m.ell_len[slice];
@@ -304,8 +312,19 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       for (int k = 0; k < len; ++k) w = fmaf(vp[k * 64], (float)(a.r[cp[k * 64]] * inv), w);
       ri = (float)(a.r[row] * inv);
       wi = w;
+      if (cont) {
+        const float cf = (float)cs;
+        si = cf * si;
+        pi = cf * pi;
+        te = (double)ri * (double)si;
+        tf = (double)pi * (double)si;
+      } else {
+        si = 0.0f;
+        pi = 0.0f;
+      }
       a.rv0[row] = ri;
       a.wv0[row] = wi;
+      a.sv1[row] = si;  // s' at the columns of iteration 0
       tg = (double)ri * (double)ri;
       td = (double)wi * (double)ri;
     }
@@ -315,9 +334,18 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       a.part_b[blockIdx.x] = s1;
       a.part_b[G + blockIdx.x] = s2;
     }
+    if (cont) {
+      const double s3 = block_sum_d(te);
+      const double s4 = block_sum_d(tf);
+      if (threadIdx.x == 0) {
+        a.part_e[blockIdx.x] = s3;
+        a.part_e[G + blockIdx.x] = s4;
+      }
+    }
     grid_publish(a.sync, ++ep);
     if (!grid_wait(a.sync, ep)) return;
-    double rho0 = 0, rho_prev = 1.0, alpha_prev = 1.0;
+    double rho0 = 0, rho_prev = rho_c, alpha_prev = 1.0;
+    cont_ok = false;
     for (int it = 0;; ++it) {
       const int cur = it & 1;
       const double* __restrict__ pc = a.part_b + (size_t)2 * G * cur;
@@ -329,10 +357,16 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       const double gamma = block_sum_d(pg);
       const double delta = block_sum_d(pd);
       if (it == 0) rho0 = gamma;
+      const bool carry = it > 0 || cont;
       double beta = 0, alpha = 0, denom = delta;
       if (it > 0) {
         beta = gamma / rho_prev;
         denom = delta - beta * gamma / alpha_prev;
+      } else if (cont) {
+        const double rs = reduce_partials(a.part_e, G);
+        const double ps = reduce_partials(a.part_e + G, G);
+        beta = gamma / rho_prev;
+        denom = fma(beta * beta, ps, fma(2.0 * beta, rs, delta));  // same rounding in every build
       }
       const bool iconv = gamma == 0 || (it > 0 && gamma < in_tol2 * rho0);
       const bool istop = iconv || it >= a.max_iter || !(denom > 0);
@@ -347,6 +381,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       }
       if (istop) {
         total_inner += it;
+        cont_ok = iconv && it > 0;
+        rho_last = it > 0 ? rho_prev : 0.0;
         break;
       }
       rho_prev = gamma;
@@ -363,8 +399,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
         const int len = a.m.ell_len[slice];
         const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
         const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
-        const float p = it > 0 ? fmaf(bf, pi, ri) : ri;
-        const float sv = it > 0 ? fmaf(bf, si, wi) : wi;
+        const float p = carry ? fmaf(bf, pi, ri) : ri;
+        const float sv = carry ? fmaf(bf, si, wi) : wi;
         const float rn = fmaf(-af, sv, ri);
         s_cur[row] = sv;
         di = fmaf(af, p, di);
@@ -377,7 +413,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
         auto term = [&](int k, float wacc) {
           const int c = k < len ? cp[k * 64] : row;
           const float v = k < len ? vp[k * 64] : 0.0f;
-          const float sk = it > 0 ? fmaf(bf, s_prev[c], w_cur[c]) : w_cur[c];
+          const float sk = carry ? fmaf(bf, s_prev[c], w_cur[c]) : w_cur[c];
           return fmaf(v, fmaf(-af, sk, r_cur[c]), wacc);
         };
         int k = 0;
@@ -387,10 +423,10 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
           const int c2 = k + 2 < len ? cp[(k + 2) * 64] : row, c3 = k + 3 < len ? cp[(k + 3) * 64] : row;
           const float v0 = k < len ? vp[k * 64] : 0.0f, v1 = k + 1 < len ? vp[(k + 1) * 64] : 0.0f;
           const float v2 = k + 2 < len ? vp[(k + 2) * 64] : 0.0f, v3 = k + 3 < len ? vp[(k + 3) * 64] : 0.0f;
-          const float s0 = it > 0 ? fmaf(bf, s_prev[c0], w_cur[c0]) : w_cur[c0];
-          const float s1v = it > 0 ? fmaf(bf, s_prev[c1], w_cur[c1]) : w_cur[c1];
-          const float s2v = it > 0 ? fmaf(bf, s_prev[c2], w_cur[c2]) : w_cur[c2];
-          const float s3 = it > 0 ? fmaf(bf, s_prev[c3], w_cur[c3]) : w_cur[c3];
+          const float s0 = carry ? fmaf(bf, s_prev[c0], w_cur[c0]) : w_cur[c0];
+          const float s1v = carry ? fmaf(bf, s_prev[c1], w_cur[c1]) : w_cur[c1];
+          const float s2v = carry ? fmaf(bf, s_prev[c2], w_cur[c2]) : w_cur[c2];
+          const float s3 = carry ? fmaf(bf, s_prev[c3], w_cur[c3]) : w_cur[c3];
           const float r0 = r_cur[c0], r1 = r_cur[c1], r2 = r_cur[c2], r3 = r_cur[c3];
           w = fmaf(v0, fmaf(-af, s0, r0), w);
           w = fmaf(v1, fmaf(-af, s1v, r1), w);
@@ -421,6 +457,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
       xi = fma(scale, (double)di, xi);
       a.x[row] = xi;
     }
+    scale_prev = scale;
     grid_publish(a.sync, ++ep);
     if (!grid_wait(a.sync, ep)) return;
   }
@@ -473,7 +510,7 @@ namespace mpic {
 hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, double tol, int max_iter,
                              double in_tol_min, unsigned* epoch, int* err, hipStream_t s) {
   if (!c.x || !c.b || !c.r || !c.rv[0] || !c.rv[1] || !c.wv[0] || !c.wv[1] || !c.sv[0] || !c.sv[1] || !c.part_a ||
-      !c.part_b || !c.st || !m32.ell_val || !epoch || !err || c.nchunks <= 0 || (int64_t)c.nchunks * BLOCK < m.nn)
+      !c.part_b || !c.part_e || !c.st || !m32.ell_val || !epoch || !err || c.nchunks <= 0 || (int64_t)c.nchunks * BLOCK < m.nn)
     return hipErrorInvalidValue;
   PIRArgs a;
   a.m = m;
@@ -489,6 +526,7 @@ hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, 
   a.sv1 = c.sv[1];
   a.part_a = c.part_a;
   a.part_b = c.part_b;
+  a.part_e = c.part_e;
   a.st = c.st;
   a.sync = GridSync{epoch, err};
   a.tol2 = tol * tol;

@@ -512,12 +512,37 @@ hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStr
   hipLaunchKernelGGL(k_field32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, phi, E);
   return hipGetLastError();
 }
+// Default grid of k_push32: six rounds of the workgroups resident at once (five
+// per CU at 88 VGPRs: 7680 on 256 CUs), capped by MAX_PARTIALS and the number
+// of 256-particle chunks.  Measured at 100M particles: 4096 workgroups 6.26 ms,
+// 5120-8192 within 1% of each other at 5.89-5.98 ms (smaller ranges balance
+// the walk's uneven cost better).
+int push32_grid(int64_t n) {
+  static int cached[16];
+  int dev = 0;
+  int g = DEFAULT_GRID;
+  if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 16) {
+    if (cached[dev] == 0) {
+      int per_cu = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push32, BLOCK, 0) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
+          cus > 0)
+        cached[dev] = std::min(MAX_PARTIALS, 6 * per_cu * cus);
+      else
+        cached[dev] = DEFAULT_GRID;
+    }
+    g = cached[dev];
+  }
+  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, chunks));
+}
+
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   if (!a.m32.erec || !a.m32.E || !a.P.r || !a.Q.r || !a.P.elem || !a.Q.elem || !a.acc || !a.partial || !a.species ||
       !a.m.bucket)
     return hipErrorInvalidValue;
-  const unsigned g = a.grid > 0 ? (unsigned)(a.grid < MAX_PARTIALS ? a.grid : MAX_PARTIALS) : grid_stride(a.n);
+  const unsigned g = a.grid > 0 ? (unsigned)(a.grid < MAX_PARTIALS ? a.grid : MAX_PARTIALS) : (unsigned)push32_grid(a.n);
   hipLaunchKernelGGL(k_push32, dim3(g), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }

@@ -294,6 +294,8 @@ struct PushArgs32 {
 };
 hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStream_t s);
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s);
+// Workgroups of k_push32 when PushArgs32::grid is 0 (= its partial count).
+int push32_grid(int64_t n);
 hipError_t launch_deposit32(const DevMesh& m, const DevParticles32& P, int64_t n, const double* species, int S,
                             unsigned long long* acc, hipStream_t s);
 // rhs[i] = boundary ? 0 : background[i] - (sum of acc over elements of node i) * 2^-S / 4.
@@ -319,8 +321,17 @@ hipError_t launch_check_owner32(const DevMesh& m, const DevParticles32& P, int64
 
 // Mixed-precision CG: FP32 CG inner solves inside FP64 iterative refinement.
 struct IRState {
-  int done, converged, outer, in_done, in_iters, total_inner, max_iter, pad;
-  double bb, rr, tol2, in_tol2, in_rho0, in_rho, scale;
+  int done, converged, outer, in_done, in_iters, total_inner, max_iter;
+  int in_ok;    // the last inner pass stopped at its tolerance
+  int cont_ok;  // ... and its direction may be continued (set between passes)
+  int cont;     // the current pass continues the previous direction
+  int in_last;  // iterations of the previous pass (parity of its last state words)
+  int pad;
+  double bb, rr, tol2, in_tol2, in_rho0, in_rho;
+  double scale;       // |r| of the current pass (x += scale * d)
+  double scale_prev;  // |r| of the previous pass (set between passes)
+  double rho_last;    // r.r of the previous pass's last completed iteration
+  double rho_c;       // rho_last in the current pass's units
 };
 struct IRBuffers {
   double *x, *b, *r;              // FP64 solution, right-hand side, residual
@@ -329,6 +340,7 @@ struct IRBuffers {
   float4* q[2];                   // the same state as (r, w, s, p) words by parity (per-iteration kernels)
   double* part_a;                 // [2*nchunks]: b.b, r.r of the outer residual
   double* part_b;                 // [4*nchunks]: r.r, w.r of the inner residual, by parity
+  double* part_e;                 // [2*nchunks]: r.s, p.s at the start of a continued pass
   double *rho_hist, *alpha_hist;  // [max_iter+1]
   IRState* st;
   int nchunks;
