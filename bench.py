@@ -141,7 +141,8 @@ def copy_calibration(nbytes=1 << 31):
 
 def run_cpu_baseline(args):
     """The FP64 oracle (restatement of the reference's OpenMP path, host solve
-    single-threaded as in the reference) on one step of the same workload."""
+    single-threaded as in the reference) on --cpu-steps steps of the same
+    workload (3 by default: about 10 s)."""
     O = load("minipic_oracle", os.path.join(ORACLE, "minipic_oracle.py"))
     import numpy as np
     threads = O.lib().orc_get_threads()
@@ -171,7 +172,7 @@ def main():
     ap.add_argument("--nppe", type=int, default=64)
     ap.add_argument("--dt", type=float, default=0.05)
     ap.add_argument("--sort-period", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run two 2 GiB device-to-device copies whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")

@@ -420,15 +420,29 @@ class DataWarehouse {
   class ES& ES_reference() { return es_; }
 
   // create_mesh (data_warehouse.cpp:50-54, mesh.cpp:78-89): the brick file holds
-  // "nx ny nz mapping"; [-1,1]^3 split into 5 tets per hex.
+  // "nx ny nz mapping"; [-1,1]^3 split into 5 tets per hex.  periodic = true joins
+  // the x = -1 and x = +1 faces (the two-stream problem's mesh; nx even): its
+  // stiffness matrix is not symmetric, so the solver becomes BiCGStab.
   void create_mesh(const std::string& in_name, bool periodic = false) {
-    if (periodic) throw minipic::Error(MPIC_ERR_ARG, "periodic meshes (the two-stream problem) are not built");
     std::ifstream in(in_name.c_str());
     int nx = 0, ny = 0, nz = 0, mapping = 0;
     if (!(in >> nx >> ny >> nz)) throw minipic::Error(MPIC_ERR_ARG, "cannot read mesh file " + in_name);
     if (!(in >> mapping)) mapping = 0;
     const double lo[3] = {-1, -1, -1}, hi[3] = {1, 1, 1};
-    create_mesh_brick(nx, ny, nz, 5, lo, hi, 0.0, 0, mapping);
+    if (periodic)
+      create_mesh_periodic_brick(nx, ny, nz, 5, lo, hi, mapping);
+    else
+      create_mesh_brick(nx, ny, nz, 5, lo, hi, 0.0, 0, mapping);
+  }
+  // Brick joined in x (mpic_mesh_brick_periodic); selects BiCGStab.
+  void create_mesh_periodic_brick(int nx, int ny, int nz, int split, const double lo[3], const double hi[3],
+                                  int mapping = 0) {
+    if (c_->mesh) throw minipic::Error(MPIC_ERR_STATE, "create_mesh called twice");
+    minipic::detail::check(mpic_mesh_brick_periodic(nx, ny, nz, split, lo, hi, mapping, &c_->mesh), nullptr,
+                           "create_mesh");
+    minipic::detail::check(mpic_mesh_describe(c_->mesh, &c_->desc), nullptr, "create_mesh");
+    c_->cfg.solver = MPIC_SOLVER_BICGSTAB;
+    mesh_ = Mesh(c_);
   }
   // Brick of nx*ny*nz hexes on [lo,hi] split into 5 (reference) or 6 (Kuhn) tets,
   // interior nodes jittered by +-jitter*h.

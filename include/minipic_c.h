@@ -54,7 +54,9 @@ typedef struct {
   int64_t nnz;
   const double* nodes;          /* [num_nodes][3]          Mesh::nodes, mesh.hpp:111 */
   const int32_t* elem_nodes;    /* [num_elems][4]          Mesh::elem_to_node_lids, mesh.hpp:136 */
-  const int32_t* elem_nbrs;     /* [num_elems][4], -1 wall Mesh::elem_face_to_elem_gids, mesh.hpp:144 */
+  const int32_t* elem_nbrs;     /* [num_elems][4]          Mesh::elem_face_to_elem_gids, mesh.hpp:144: the neighbour
+                                   across face f, -1 at a wall, -2-e for neighbour e across a periodic face
+                                   (Mesh::elem_face_periodic, mesh.cpp:376-389) */
   const double* normals;        /* [num_elems][4][3]       Mesh::normals, mesh.hpp:112 */
   const double* inv_jacobian;   /* [num_elems][3][3]       Mesh::inverse_jacobian(e,0), mesh.hpp:153 */
   const double* det_jacobian;   /* [num_elems]             Mesh::determinate_jacobian, mesh.hpp:154 */
@@ -65,6 +67,8 @@ typedef struct {
   const int32_t* col_idx;       /* [nnz], ascending within a row */
   const double* values;         /* [nnz] */
   const double* background;     /* [num_nodes] constant rhs term (ES::set_analytic_RHS, ES.cpp:235-257) or NULL */
+  const int32_t* node_pair;     /* [num_nodes] or NULL: for a periodic image node, the node whose row carries its
+                                   charge and stiffness (Mesh::node_pairs, mesh.cpp:178-189), else -1 */
 } mpic_mesh_desc;
 
 typedef struct {
@@ -128,6 +132,17 @@ const char* mpic_last_error(const mpic_ctx* ctx); /* ctx may be NULL: last creat
  * the reference's mesh file field (mesh.cpp:85-89,763-792). */
 int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
                     uint64_t jitter_seed, int mapping, mpic_mesh** out);
+/* The same brick, periodic in x (the two-stream problem's mesh, Mesh::Mesh with
+ * periodic = true, mesh.cpp:91-94,178-189,365-412): nx must be even; the x = lo
+ * and x = hi faces are joined (a particle crossing one moves to the other with
+ * x -> lo + hi - x, particle_move.hpp:190-191); nodes on x = hi (away from the
+ * y and z walls) are images of their x = lo partners, whose rows take their
+ * charge and stiffness (ES.cpp:91-110,168-189; PIC.cpp:132-134); an image row
+ * is -phi_image + phi_partner = 0.  Dirichlet nodes: the y and z walls only.
+ * The stiffness matrix is then not symmetric: solve with MPIC_SOLVER_BICGSTAB
+ * (the reference's solver) in the FP64 form. */
+int mpic_mesh_brick_periodic(int nx, int ny, int nz, int split, const double* lo, const double* hi, int mapping,
+                             mpic_mesh** out);
 /* Uniform background charge density f as an rhs load vector (ES::set_analytic_RHS with a constant). */
 int mpic_mesh_set_background(mpic_mesh* mesh, double f);
 /* Load vector of a source density f: out[i] = sum over elements and cubature points c of

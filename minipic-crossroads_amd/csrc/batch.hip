@@ -195,6 +195,8 @@ int mpic_batch_create(int nprob, mpic_mesh* const* meshes, const mpic_config* cf
   for (int p = 0; p < nprob; ++p) {
     if (!meshes[p] || mpic_mesh_describe(meshes[p], &d[p]) != MPIC_OK)
       return bfail(nullptr, MPIC_ERR_ARG, "mpic_batch_create: bad mesh");
+    if (d[p].node_pair)  // the batch solve is the precision form's CG
+      return bfail(nullptr, MPIC_ERR_ARG, "mpic_batch_create: periodic meshes need BiCGStab (single contexts)");
     nn += d[p].num_nodes;
     ne += d[p].num_elems;
     nnz += d[p].nnz;

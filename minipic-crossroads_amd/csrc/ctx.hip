@@ -401,8 +401,10 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     sixodet[e] = 6 / d->det_jacobian[e];  // the (6/ detJ) of PIC.cpp:125
     for (int k = 0; k < 4; ++k) {
       const int32_t v = d->elem_nodes[e * 4 + k], b = d->elem_nbrs[e * 4 + k];
-      if (v < 0 || v >= nn || b < -1 || b >= ne)
+      // b: neighbour, -1 wall, -2 - neighbour across a periodic face
+      if (v < 0 || v >= nn || b >= ne || (b < -1 && -2 - (int64_t)b >= ne))
         return bail(fail(c, MPIC_ERR_ARG, "mpic_create: connectivity out of range"));
+      if (b < -1) c->periodic = true;
       conn[e * 4 + k] = v;
       nbrs[e * 4 + k] = b;
     }
@@ -411,13 +413,30 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     std::memcpy(&erec[e * 16 + 12], &nbrs[e * 4], 4 * sizeof(int32_t));
     erec[e * 16 + 14] = sixodet[e];
   }
-  // Node -> element slot adjacency (slot = e*4 + k), element order.
+  // Node -> element slot adjacency (slot = e*4 + k), element order.  The charge
+  // of a periodic image node goes to its partner's row (PIC.cpp:132-134): the
+  // partner gathers the image's slots, the image none.
+  std::vector<int32_t> row_of(conn);
+  if (d->node_pair) {
+    for (int32_t i = 0; i < nn; ++i)
+      if (d->node_pair[i] >= nn || (d->node_pair[i] >= 0 && d->node_pair[d->node_pair[i]] >= 0))
+        return bail(fail(c, MPIC_ERR_ARG, "mpic_create: node pair out of range"));
+    for (auto& v : row_of)
+      if (d->node_pair[v] >= 0) {
+        v = d->node_pair[v];
+        c->periodic = true;
+      }
+  }
+  if (c->periodic && c->cfg.solver != MPIC_SOLVER_BICGSTAB)
+    return bail(fail(c, MPIC_ERR_ARG,
+                     "mpic_create: a periodic mesh has a non-symmetric stiffness matrix; use MPIC_SOLVER_BICGSTAB "
+                     "(the reference's solver) in the FP64 form"));
   std::vector<int32_t> adj_ptr(nn + 1, 0), adj((size_t)ne * 4);
-  for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj_ptr[conn[s] + 1]++;
+  for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj_ptr[row_of[s] + 1]++;
   for (int32_t i = 0; i < nn; ++i) adj_ptr[i + 1] += adj_ptr[i];
   {
     std::vector<int32_t> pos(adj_ptr.begin(), adj_ptr.end() - 1);
-    for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj[pos[conn[s]]++] = (int32_t)s;
+    for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj[pos[row_of[s]]++] = (int32_t)s;
   }
   // Sliced ELL from CSR.
   const int32_t nsl = (nn + ELL_SLICE - 1) / ELL_SLICE;
@@ -513,6 +532,9 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       c->m.lo[a] = std::min(c->m.lo[a], d->nodes[(size_t)i * 3 + a]);
       c->m.hi[a] = std::max(c->m.hi[a], d->nodes[(size_t)i * 3 + a]);
     }
+  // A particle crossing a periodic face moves to the other side: x -> lo + hi - x
+  // (x -> -x on the reference's [-1,1] box, particle_move.hpp:190-191).
+  c->m.period_x = c->m.lo[0] + c->m.hi[0];
   {
     int32_t nb = 0;
     std::vector<int32_t> bk = make_buckets(d, conn, c->cfg.sort_bucket, c->m.lo, c->m.hi, &nb);

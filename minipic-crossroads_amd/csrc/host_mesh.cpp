@@ -26,6 +26,10 @@ struct mpic_mesh {
   std::vector<int32_t> elem_nodes, nbrs, row_ptr, col;
   std::vector<uint8_t> boundary;
   bool has_background = false;
+  bool periodic = false;             // joined in x (the two-stream mesh)
+  std::vector<int32_t> node_pair;    // periodic: image node -> partner on x = lo, else -1
+  // The row that takes node n's charge and stiffness (Mesh::node_pairs).
+  int32_t row_of(int32_t n) const { return periodic && node_pair[n] >= 0 ? node_pair[n] : n; }
 };
 
 namespace {
@@ -80,6 +84,7 @@ class BrickBuilder {
 
   bool run(double jitter, uint64_t jseed, int mapping) {
     make_nodes(jitter, jseed);
+    if (m_.periodic) make_pairs();
     if (mapping != 0)
       for (int32_t g = 0; g < m_.nn; ++g) map_node(&m_.nodes[(size_t)g * 3], mapping);
     make_elements();
@@ -112,7 +117,8 @@ class BrickBuilder {
           bool on_wall = false;
           for (int a = 0; a < 3; ++a) {
             x[a] = m_.lo[a] + idx[a] * h[a];  // -1 + i*dx in mesh.cpp:141
-            on_wall = on_wall || idx[a] == 0 || idx[a] == n[a];
+            // the x walls are joined, not Dirichlet, in a periodic mesh (mesh.cpp:410)
+            on_wall = on_wall || ((a > 0 || !m_.periodic) && (idx[a] == 0 || idx[a] == n[a]));
           }
           m_.boundary[g] = on_wall;  // nodeset, mesh.cpp:408-412
           if (on_wall || jitter == 0.0) continue;
@@ -123,6 +129,14 @@ class BrickBuilder {
             x[a] = x[a] + (jitter * h[a]) * (2.0 * u - 1.0);
           }
         }
+  }
+
+  // Mesh::node_pairs (mesh.cpp:178-189): a node on x = hi away from the y and z
+  // walls is the image of the node on x = lo with the same (j, k).
+  void make_pairs() {
+    m_.node_pair.assign(m_.nn, -1);
+    for (int j = 1; j < m_.ny; ++j)
+      for (int k = 1; k < m_.nz; ++k) m_.node_pair[node_id(m_.nx, j, k)] = (int32_t)node_id(0, j, k);
   }
 
   void make_elements() {
@@ -142,14 +156,24 @@ class BrickBuilder {
   }
 
   // Faces shared by two elements are neighbours; unmatched faces are walls (-1).
+  // Periodic: a face with all three nodes on x = hi is keyed by its x = lo
+  // images (Mesh::periodic_face, mesh.cpp:62-76), so it matches the face across
+  // the period; such pairs are stored as -2 - neighbour (mesh.cpp:376-389).
   void match_faces() {
     const int64_t ne = m_.ne;
     const uint64_t nn = (uint64_t)m_.nn;
+    const int64_t plane = (int64_t)(m_.ny + 1) * (m_.nz + 1);  // nodes per x plane; i = id / plane
     std::vector<std::pair<uint64_t, int64_t>> keys(ne * 4);
     for (int64_t e = 0; e < ne; ++e)
       for (int f = 0; f < 4; ++f) {
         uint64_t v[3];
-        for (int c = 0; c < 3; ++c) v[c] = (uint64_t)m_.elem_nodes[e * 4 + kFaces[f][c]];
+        bool on_hi = m_.periodic;
+        for (int c = 0; c < 3; ++c) {
+          v[c] = (uint64_t)m_.elem_nodes[e * 4 + kFaces[f][c]];
+          on_hi = on_hi && (int64_t)v[c] / plane == m_.nx;
+        }
+        if (on_hi)
+          for (int c = 0; c < 3; ++c) v[c] -= (uint64_t)(m_.nx * plane);
         std::sort(v, v + 3);
         keys[e * 4 + f] = {(v[0] * nn + v[1]) * nn + v[2], e * 4 + f};
       }
@@ -157,8 +181,11 @@ class BrickBuilder {
     m_.nbrs.assign(ne * 4, -1);
     for (size_t a = 0; a + 1 < keys.size(); ++a)
       if (keys[a].first == keys[a + 1].first) {
-        m_.nbrs[keys[a].second] = (int32_t)(keys[a + 1].second / 4);
-        m_.nbrs[keys[a + 1].second] = (int32_t)(keys[a].second / 4);
+        const int32_t e0 = (int32_t)(keys[a].second / 4), e1 = (int32_t)(keys[a + 1].second / 4);
+        // joined across the period: the largest node id of the key is on x = lo
+        const bool across = m_.periodic && (int64_t)(keys[a].first % nn) < plane;
+        m_.nbrs[keys[a].second] = across ? -2 - e1 : e1;
+        m_.nbrs[keys[a + 1].second] = across ? -2 - e0 : e0;
         ++a;
       }
   }
@@ -230,18 +257,21 @@ class BrickBuilder {
   void assemble() {
     const int64_t nn = m_.nn, ne = m_.ne;
     // Row patterns: interior rows take every node of every touching element;
-    // Dirichlet rows only their diagonal (ES.cpp:91-101).
+    // Dirichlet rows only their diagonal (ES.cpp:91-101).  Periodic: rows and
+    // columns of image nodes go to their partners; an image row holds itself and
+    // its partner (ES.cpp:104-110).
     std::vector<std::vector<int32_t>> pattern(nn);
     for (int64_t e = 0; e < ne; ++e)
       for (int a = 0; a < 4; ++a) {
-        const int32_t row = m_.elem_nodes[e * 4 + a];
+        const int32_t row = m_.row_of(m_.elem_nodes[e * 4 + a]);
         if (m_.boundary[row]) continue;
-        for (int b = 0; b < 4; ++b) pattern[row].push_back(m_.elem_nodes[e * 4 + b]);
+        for (int b = 0; b < 4; ++b) pattern[row].push_back(m_.row_of(m_.elem_nodes[e * 4 + b]));
       }
     m_.row_ptr.assign(nn + 1, 0);
     for (int64_t r = 0; r < nn; ++r) {
       auto& p = pattern[r];
       if (m_.boundary[r]) p.assign(1, (int32_t)r);
+      if (m_.row_of((int32_t)r) != r) p = {(int32_t)r, m_.row_of((int32_t)r)};
       std::sort(p.begin(), p.end());
       p.erase(std::unique(p.begin(), p.end()), p.end());
       m_.row_ptr[r + 1] = m_.row_ptr[r] + (int32_t)p.size();
@@ -255,27 +285,36 @@ class BrickBuilder {
       double K[4][4];
       element_matrix(e, K);
       for (int a = 0; a < 4; ++a) {
-        const int32_t row = m_.elem_nodes[e * 4 + a];
+        const int32_t row = m_.row_of(m_.elem_nodes[e * 4 + a]);
         if (m_.boundary[row]) continue;
         const int32_t* first = m_.col.data() + m_.row_ptr[row];
         const int32_t* last = m_.col.data() + m_.row_ptr[row + 1];
         for (int b = 0; b < 4; ++b) {
-          const int64_t at = std::lower_bound(first, last, m_.elem_nodes[e * 4 + b]) - m_.col.data();
+          const int64_t at = std::lower_bound(first, last, m_.row_of(m_.elem_nodes[e * 4 + b])) - m_.col.data();
           m_.val[at] += K[a][b];
         }
       }
     }
-    for (int64_t r = 0; r < nn; ++r)
+    for (int64_t r = 0; r < nn; ++r) {
       if (m_.boundary[r]) m_.val[m_.row_ptr[r]] = 1.0;  // ES.cpp:198-204
+      const int32_t pr = m_.row_of((int32_t)r);
+      if (pr != r)  // image row: -phi_r + phi_partner (ES.cpp:181-186)
+        for (int32_t at = m_.row_ptr[r]; at < m_.row_ptr[r + 1]; ++at) m_.val[at] = m_.col[at] == r ? -1.0 : 1.0;
+    }
   }
 };
 
 }  // namespace
 
-extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
-                               uint64_t jitter_seed, int mapping, mpic_mesh** out) {
+namespace {
+int make_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
+               uint64_t jitter_seed, int mapping, bool periodic, mpic_mesh** out) {
   if (!out || !lo || !hi || nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6) || mapping < 0 || mapping > 3) {
     mpic::global_error() = "mpic_mesh_brick: bad arguments";
+    return MPIC_ERR_ARG;
+  }
+  if (periodic && nx % 2 != 0) {  // mesh.cpp:91-94
+    mpic::global_error() = "mpic_mesh_brick_periodic: a periodic mesh needs an even number of hexes in x";
     return MPIC_ERR_ARG;
   }
   const double nodes = (double)(nx + 1) * (ny + 1) * (nz + 1);
@@ -284,6 +323,7 @@ extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* 
     return MPIC_ERR_ARG;
   }
   mpic_mesh* m = new mpic_mesh();
+  m->periodic = periodic;
   m->nx = nx;
   m->ny = ny;
   m->nz = nz;
@@ -300,6 +340,17 @@ extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* 
   *out = m;
   return MPIC_OK;
 }
+}  // namespace
+
+extern "C" int mpic_mesh_brick(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
+                               uint64_t jitter_seed, int mapping, mpic_mesh** out) {
+  return make_brick(nx, ny, nz, split, lo, hi, jitter, jitter_seed, mapping, false, out);
+}
+
+extern "C" int mpic_mesh_brick_periodic(int nx, int ny, int nz, int split, const double* lo, const double* hi,
+                                        int mapping, mpic_mesh** out) {
+  return make_brick(nx, ny, nz, split, lo, hi, 0.0, 0, mapping, true, out);
+}
 
 extern "C" int mpic_mesh_set_background(mpic_mesh* m, double f) {
   if (!m) return MPIC_ERR_ARG;
@@ -308,9 +359,9 @@ extern "C" int mpic_mesh_set_background(mpic_mesh* m, double f) {
     for (int i = 0; i < 4; ++i) {
       double v = 0;
       for (int c = 0; c < 4; ++c) v += -f * m->wbv[e * 16 + i * 4 + c];
-      const int32_t row = m->elem_nodes[e * 4 + i];
-      if (m->boundary[row]) v = 0;
-      m->background[row] += v;
+      const int32_t node = m->elem_nodes[e * 4 + i];
+      if (m->boundary[node]) v = 0;
+      m->background[m->row_of(node)] += v;  // ES.cpp:247-251
     }
   m->has_background = true;
   return MPIC_OK;
@@ -339,9 +390,9 @@ extern "C" int mpic_mesh_load_vector(const mpic_mesh* m, double (*f)(const doubl
     for (int i = 0; i < 4; ++i) {
       double v = 0;
       for (int c = 0; c < 4; ++c) v += -fc[c] * m->wbv[e * 16 + i * 4 + c];
-      const int32_t row = m->elem_nodes[e * 4 + i];
-      if (m->boundary[row]) v = 0;
-      out[row] += v;
+      const int32_t node = m->elem_nodes[e * 4 + i];
+      if (m->boundary[node]) v = 0;
+      out[m->row_of(node)] += v;  // ES.cpp:247-251
     }
   }
   return MPIC_OK;
@@ -365,6 +416,7 @@ extern "C" int mpic_mesh_describe(const mpic_mesh* m, mpic_mesh_desc* d) {
   d->col_idx = m->col.data();
   d->values = m->val.data();
   d->background = m->has_background ? m->background.data() : nullptr;
+  d->node_pair = m->periodic ? m->node_pair.data() : nullptr;
   return MPIC_OK;
 }
 

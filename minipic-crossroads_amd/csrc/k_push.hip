@@ -150,8 +150,10 @@ __device__ __forceinline__ bool walk(const DevMesh& m, int& e, ElemRec& g, doubl
       R[d] = R[d] + s * ldt * VR[d];
     }
     if (s < 1) {
-      const int e2 = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
-      if (e2 >= 0) {
+      const int nbv = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
+      if (nbv != -1) {  // a neighbour (-2 - e: across the period, particle_move.hpp:190-191)
+        const int e2 = nbv >= 0 ? nbv : -2 - nbv;
+        if (nbv < -1) X[0] = m.period_x - X[0];
         e = e2;
         wc.crossings++;
         g = load_erec(m.erec, e);  // inverse Jacobian, node 0 and neighbours of the new element

@@ -8,7 +8,8 @@
 //   --tfinal T        final time; steps while t < tfinal, t += dt (2.0)
 //   --output K        write output.<t> every K steps (0 = never)
 //   --kill-particles 1  remove particles that hit a wall
-//   --problem NAME    box | none (e- and e+ fills, temp 1) | plasma (see below); two-stream is not built
+//   --problem NAME    box | none (e- and e+ fills, temp 1) | two-stream (mesh periodic in x, two
+//                     counter-streaming electron beams and ions, BiCGStab at 1e-6) | plasma (see below)
 //   --density D       fill density                             (2 pi^2 / 3)
 //   --dump_mesh 1     write the mesh as text
 // Options of the configurations this library is measured on:
@@ -58,7 +59,7 @@ struct Options {
 const char* kUsage =
     "usage: minipic_main [--nparts N] [--mesh FILE | --nx N --ny N --nz N [--split 5|6] [--unit-cube 1]]\n"
     "                    [--dt DT] [--tfinal T | --steps N] [--output K] [--kill-particles 1]\n"
-    "                    [--problem box|none|plasma] [--density D] [--dump_mesh 1]\n"
+    "                    [--problem box|none|two-stream|plasma] [--density D] [--dump_mesh 1]\n"
     "                    [--form fp64|mixed] [--cg-tol TOL] [--solver cg|bicgstab] [--sort-period N]\n"
     "                    [--vth V] [--pert A] [--drift U] [--seed S] [--jitter J] [--jitter-seed S]\n"
     "                    [--capacity-factor F] [--quiet 1]\n";
@@ -88,7 +89,7 @@ int run(const Options& o) {
   const double density = o.num("density", 2. / 3. * M_PI * M_PI);
   const int output = (int)o.integer("output", 0);
   const bool kill = o.integer("kill-particles", 0) == 1;
-  if (problem == "two-stream") throw minipic::Error(MPIC_ERR_ARG, "the periodic two-stream problem is not built");
+  const bool two_stream = problem == "two-stream";
 
   DataWarehouse data;
   if (o.has("nx")) {
@@ -96,15 +97,19 @@ int run(const Options& o) {
     const bool unit = o.integer("unit-cube", 0) == 1;
     const double lo[3] = {unit ? 0.0 : -1.0, unit ? 0.0 : -1.0, unit ? 0.0 : -1.0};
     const double hi[3] = {1.0, 1.0, 1.0};
-    data.create_mesh_brick(nx, ny, nz, (int)o.integer("split", 5), lo, hi, o.num("jitter", 0.0),
-                           (uint64_t)o.integer("jitter-seed", 0), 0);
+    if (two_stream)
+      data.create_mesh_periodic_brick(nx, ny, nz, (int)o.integer("split", 5), lo, hi, 0);
+    else
+      data.create_mesh_brick(nx, ny, nz, (int)o.integer("split", 5), lo, hi, o.num("jitter", 0.0),
+                             (uint64_t)o.integer("jitter-seed", 0), 0);
   } else {
-    data.create_mesh(o.get("mesh", "brick.txt"));
+    data.create_mesh(o.get("mesh", "brick.txt"), two_stream);  // main.cpp:89-92
   }
   mpic_config& cfg = data.config();
   cfg.form = o.get("form", "fp64") == "mixed" ? MPIC_FORM_MIXED : MPIC_FORM_FP64;
-  cfg.cg_tol = o.num("cg-tol", cfg.cg_tol);
-  cfg.solver = o.get("solver", "cg") == "bicgstab" ? MPIC_SOLVER_BICGSTAB : MPIC_SOLVER_CG;
+  // the two-stream problem runs the reference's solver: BiCGStab at 1e-6 (PIC.cpp:251)
+  cfg.cg_tol = o.num("cg-tol", two_stream ? 1e-6 : cfg.cg_tol);
+  cfg.solver = two_stream || o.get("solver", "cg") == "bicgstab" ? MPIC_SOLVER_BICGSTAB : MPIC_SOLVER_CG;
   cfg.sort_period = (int)o.integer("sort-period", cfg.sort_period);
   const bool plasma = problem == "plasma";
   if (plasma) data.set_background_density(density);
@@ -114,7 +119,15 @@ int run(const Options& o) {
   data.set_kill_particles_on_wall_impact(kill);
 
   const auto t0 = std::chrono::steady_clock::now();
-  if (plasma) {
+  if (two_stream) {  // main.cpp:98-106
+    VECTOR v(3, 0.0);
+    v[0] = 0.5;
+    const FLOAT temp = -.05;
+    ParticleFill fill1a(data, nppe / 2, 0, density / 2, v, temp);
+    v[0] *= -1;
+    ParticleFill fill1b(data, nppe / 2, 0, density / 2, v, temp);
+    ParticleFill fill2(data, nppe, 2, density, VECTOR(3, 0.0), 0);
+  } else if (plasma) {
     const double vth = o.num("vth", 0.1), drift = o.num("drift", 0.0), pert = o.num("pert", 0.0);
     const uint64_t seed = (uint64_t)o.integer("seed", 1);
     if (drift != 0.0) {

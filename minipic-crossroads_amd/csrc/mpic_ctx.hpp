@@ -34,6 +34,7 @@ struct mpic_ctx {
   std::vector<void*> allocs;  // mesh-lifetime device buffers
 
   DevMesh m{};
+  bool periodic = false;  // the mesh joins faces across a period (BiCGStab, FP64 form only)
   std::vector<double> background;  // host copy
   double bg_abs_sum = 0;
   fx128* bgfix = nullptr;

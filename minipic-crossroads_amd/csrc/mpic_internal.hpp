@@ -63,6 +63,7 @@ struct DevMesh {
   const int32_t* ell_col;
   const double* ell_val;
   double lo[3], hi[3];
+  double period_x;  // lo[0] + hi[0]: x -> period_x - x across a periodic face (neighbour stored as -2 - e)
 };
 
 // Particles on the device are records (one 16-B aligned record per particle,

@@ -20,7 +20,8 @@ MAX_SPECIES = 16
 
 # Every extern "C" symbol declared in include/minipic_c.h.
 EXPORTS = [
-    "mpic_config_default", "mpic_last_error", "mpic_mesh_brick", "mpic_mesh_set_background", "mpic_mesh_describe",
+    "mpic_config_default", "mpic_last_error", "mpic_mesh_brick", "mpic_mesh_brick_periodic", "mpic_mesh_set_background",
+    "mpic_mesh_describe",
     "mpic_mesh_free", "mpic_create", "mpic_destroy", "mpic_set_form", "mpic_particles_reserve",
     "mpic_particles_upload", "mpic_particles_download", "mpic_num_particles", "mpic_fill", "mpic_weight_charge",
     "mpic_solve", "mpic_weight_efield", "mpic_accel", "mpic_move", "mpic_compute_ke", "mpic_sort_by_element",
@@ -39,7 +40,7 @@ class MeshDesc(C.Structure):
                 ("normals", C.c_void_p), ("inv_jacobian", C.c_void_p), ("det_jacobian", C.c_void_p),
                 ("grad_basis", C.c_void_p), ("wbv", C.c_void_p), ("boundary", C.c_void_p),
                 ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("values", C.c_void_p),
-                ("background", C.c_void_p)]
+                ("background", C.c_void_p), ("node_pair", C.c_void_p)]
 
 
 class Config(C.Structure):
@@ -97,6 +98,7 @@ def lib():
             "mpic_config_default": (None, [P]),
             "mpic_last_error": (C.c_char_p, [P]),
             "mpic_mesh_brick": (i32, [i32, i32, i32, i32, P, P, d, u64, i32, P]),
+            "mpic_mesh_brick_periodic": (i32, [i32, i32, i32, i32, P, P, i32, P]),
             "mpic_mesh_set_background": (i32, [P, d]),
             "mpic_mesh_describe": (i32, [P, P]),
             "mpic_mesh_free": (None, [P]),
@@ -171,12 +173,15 @@ class Mesh:
     """Host mesh + discretisation built by the library (Mesh/ES/PIC setup)."""
 
     def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0,
-                 background=None, mapping=0):
+                 background=None, mapping=0, periodic=False):
         self._lo = np.array(lo, dtype=np.float64)
         self._hi = np.array(hi, dtype=np.float64)
         h = C.c_void_p()
-        rc = lib().mpic_mesh_brick(nx, ny, nz, split, _p(self._lo), _p(self._hi), jitter, jitter_seed, mapping,
-                                   C.byref(h))
+        if periodic:
+            rc = lib().mpic_mesh_brick_periodic(nx, ny, nz, split, _p(self._lo), _p(self._hi), mapping, C.byref(h))
+        else:
+            rc = lib().mpic_mesh_brick(nx, ny, nz, split, _p(self._lo), _p(self._hi), jitter, jitter_seed, mapping,
+                                       C.byref(h))
         if rc != OK:
             raise MpicError(rc, lib().mpic_last_error(None).decode())
         self.h = h
@@ -193,7 +198,8 @@ class Mesh:
                 "normals": (C.c_double, (ne, 4, 3)), "inv_jacobian": (C.c_double, (ne, 3, 3)),
                 "det_jacobian": (C.c_double, (ne,)), "grad_basis": (C.c_double, (ne, 4, 3)),
                 "wbv": (C.c_double, (ne, 4, 4)), "boundary": (C.c_uint8, (nn,)), "row_ptr": (C.c_int32, (nn + 1,)),
-                "col_idx": (C.c_int32, (nnz,)), "values": (C.c_double, (nnz,)), "background": (C.c_double, (nn,))}
+                "col_idx": (C.c_int32, (nnz,)), "values": (C.c_double, (nnz,)), "background": (C.c_double, (nn,)),
+                "node_pair": (C.c_int32, (nn,))}
         ct, shape = spec[name]
         ptr = getattr(self.desc, name)
         if not ptr:

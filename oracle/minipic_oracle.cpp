@@ -88,6 +88,13 @@ struct Mesh {
   std::vector<uint8_t> bnd;     // [nn]
   std::vector<int32_t> rowptr, col;
   std::vector<double> val;
+  // Periodic in x (Mesh::Mesh with periodic = true): image node -> partner on
+  // x = lo (Mesh::node_pairs, mesh.cpp:178-189), else -1; faces joined across
+  // the period are stored in nb as -2 - neighbour; px = lo[0] + hi[0].
+  bool periodic = false;
+  std::vector<int32_t> pair;
+  double px = 0.0;
+  int32_t row_of(int32_t n) const { return periodic && pair[n] >= 0 ? pair[n] : n; }
 };
 
 inline double sq_sum3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -143,7 +150,8 @@ void build_mesh(Mesh& m, double jitter, uint64_t jseed, int mapping) {
         p[0] = m.lo[0] + i * d[0];
         p[1] = m.lo[1] + j * d[1];
         p[2] = m.lo[2] + k * d[2];
-        bool b = (i == 0 || i == nx || j == 0 || j == ny || k == 0 || k == nz);  // mesh.cpp:408-411
+        // mesh.cpp:408-411; the x walls of a periodic mesh are joined, not Dirichlet (:410)
+        bool b = ((!m.periodic && (i == 0 || i == nx)) || j == 0 || j == ny || k == 0 || k == nz);
         m.bnd[g] = b;
         if (!b && jitter != 0.0) {
           // Seeded interior jitter, uniform in [-jitter*d, +jitter*d] per axis.
@@ -156,6 +164,13 @@ void build_mesh(Mesh& m, double jitter, uint64_t jseed, int mapping) {
         }
       }
   for (int64_t g = 0; g < nn; ++g) apply_mapping(m.nodes[g * 3], m.nodes[g * 3 + 1], m.nodes[g * 3 + 2], mapping);
+  const int64_t plane = (int64_t)(ny + 1) * (nz + 1);  // node id = i*plane + j*(nz+1) + k
+  if (m.periodic) {  // mesh.cpp:178-189
+    m.pair.assign(nn, -1);
+    for (int j = 1; j < ny; ++j)
+      for (int k = 1; k < nz; ++k) m.pair[(int64_t)nx * plane + (int64_t)j * (nz + 1) + k] = (int32_t)((int64_t)j * (nz + 1) + k);
+    m.px = m.lo[0] + m.hi[0];
+  }
   // Elements, mesh.cpp:196-224 (5-tet) or the Kuhn 6-tet split.
   m.en.assign(ne * 4, 0);
   int64_t cnt = 0;
@@ -177,6 +192,13 @@ void build_mesh(Mesh& m, double jitter, uint64_t jseed, int mapping) {
   for (int64_t e = 0; e < ne; ++e)
     for (int f = 0; f < 4; ++f) {
       uint64_t a = m.en[e * 4 + FACE[f][0]], b = m.en[e * 4 + FACE[f][1]], c = m.en[e * 4 + FACE[f][2]];
+      // Mesh::periodic_face (mesh.cpp:62-76): a face on x = hi is keyed by its x = lo nodes
+      const uint64_t shift = (uint64_t)nx * (uint64_t)plane;
+      if (m.periodic && (int64_t)a / plane == nx && (int64_t)b / plane == nx && (int64_t)c / plane == nx) {
+        a -= shift;
+        b -= shift;
+        c -= shift;
+      }
       if (a > b) std::swap(a, b);
       if (b > c) std::swap(b, c);
       if (a > b) std::swap(a, b);
@@ -189,8 +211,11 @@ void build_mesh(Mesh& m, double jitter, uint64_t jseed, int mapping) {
     while (r < faces.size() && faces[r].first == faces[q].first) ++r;
     if (r - q == 2) {
       int64_t f0 = faces[q].second, f1 = faces[q + 1].second;
-      m.nb[f0] = (int32_t)(f1 / 4);
-      m.nb[f1] = (int32_t)(f0 / 4);
+      // a matched face whose nodes all lie on x = lo joins the two sides of the
+      // period (mesh.cpp:376-389: elem_face_periodic)
+      const bool across = m.periodic && (int64_t)(faces[q].first % (uint64_t)nn) < plane;
+      m.nb[f0] = across ? -2 - (int32_t)(f1 / 4) : (int32_t)(f1 / 4);
+      m.nb[f1] = across ? -2 - (int32_t)(f0 / 4) : (int32_t)(f0 / 4);
     }
     q = r;
   }
@@ -273,16 +298,18 @@ void elem_stiffness(const Mesh& m, int64_t e, double K[4][4]) {
 // element touching the row (boundary columns kept); boundary rows are identity.
 // Columns ascend within a row; each entry sums element contributions in element order.
 void build_csr(Mesh& m) {
+  // Periodic: rows and columns of image nodes go to their partners; an image
+  // row is (-1 at itself, +1 at its partner), ES.cpp:91-110,168-189.
   const int64_t nn = m.nn, ne = m.ne;
   std::vector<int64_t> ecount(nn + 1, 0);
   for (int64_t e = 0; e < ne; ++e)
-    for (int c = 0; c < 4; ++c) ecount[m.en[e * 4 + c] + 1]++;
+    for (int c = 0; c < 4; ++c) ecount[m.row_of(m.en[e * 4 + c]) + 1]++;
   for (int64_t i = 0; i < nn; ++i) ecount[i + 1] += ecount[i];
   std::vector<int32_t> elist(ecount[nn]);
   {
     std::vector<int64_t> pos(ecount.begin(), ecount.end() - 1);
     for (int64_t e = 0; e < ne; ++e)
-      for (int c = 0; c < 4; ++c) elist[pos[m.en[e * 4 + c]]++] = (int32_t)e;
+      for (int c = 0; c < 4; ++c) elist[pos[m.row_of(m.en[e * 4 + c])]++] = (int32_t)e;
   }
   std::vector<std::vector<int32_t>> cols(nn);
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -292,8 +319,13 @@ void build_csr(Mesh& m) {
       cl.push_back((int32_t)i);
       continue;
     }
+    if (m.row_of((int32_t)i) != i) {
+      cl.push_back(m.row_of((int32_t)i));
+      cl.push_back((int32_t)i);
+      continue;
+    }
     for (int64_t q = ecount[i]; q < ecount[i + 1]; ++q)
-      for (int c = 0; c < 4; ++c) cl.push_back(m.en[(int64_t)elist[q] * 4 + c]);
+      for (int c = 0; c < 4; ++c) cl.push_back(m.row_of(m.en[(int64_t)elist[q] * 4 + c]));
     std::sort(cl.begin(), cl.end());
     cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
   }
@@ -306,10 +338,10 @@ void build_csr(Mesh& m) {
     double K[4][4];
     elem_stiffness(m, e, K);
     for (int a = 0; a < 4; ++a) {
-      int32_t row = m.en[e * 4 + a];
+      int32_t row = m.row_of(m.en[e * 4 + a]);
       if (m.bnd[row]) continue;  // ES.cpp:178-180
       for (int b = 0; b < 4; ++b) {
-        int32_t cc = m.en[e * 4 + b];
+        int32_t cc = m.row_of(m.en[e * 4 + b]);
         int32_t* beg = &m.col[m.rowptr[row]];
         int32_t* end = &m.col[m.rowptr[row + 1]];
         int64_t p = std::lower_bound(beg, end, cc) - &m.col[0];
@@ -317,8 +349,11 @@ void build_csr(Mesh& m) {
       }
     }
   }
-  for (int64_t i = 0; i < nn; ++i)
+  for (int64_t i = 0; i < nn; ++i) {
     if (m.bnd[i]) m.val[m.rowptr[i]] = 1.0;  // ES.cpp:198-204
+    if (m.row_of((int32_t)i) != i)          // image row, ES.cpp:181-186
+      for (int32_t q = m.rowptr[i]; q < m.rowptr[i + 1]; ++q) m.val[q] = m.col[q] == i ? -1.0 : 1.0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -510,7 +545,11 @@ inline bool walk(const Mesh& m, int32_t& e, double x[3], double xr[3], double v[
     }
     if (s < 1) {
       int32_t ne2 = m.nb[(int64_t)e * 4 + face];
-      if (ne2 >= 0) {
+      if (ne2 != -1) {
+        if (ne2 < -1) {  // across the period: x[0] = -x[0] on [-1,1], particle_move.hpp:190-191
+          ne2 = -2 - ne2;
+          x[0] = m.px - x[0];
+        }
         e = ne2;
         st.crossings++;
         // physToReference, mesh.hpp:197-211
@@ -551,9 +590,11 @@ void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int orc_get_threads() { return omp_get_max_threads(); }
 
 void* orc_mesh_create(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
-                      uint64_t jseed, int mapping) {
+                      uint64_t jseed, int mapping, int periodic) {
   if (nx < 1 || ny < 1 || nz < 1 || (split != 5 && split != 6) || mapping < 0 || mapping > 3) return nullptr;
+  if (periodic && (nx % 2 != 0 || jitter != 0.0)) return nullptr;  // mesh.cpp:91-94
   Mesh* m = new Mesh();
+  m->periodic = periodic != 0;
   m->nx = nx;
   m->ny = ny;
   m->nz = nz;
@@ -603,6 +644,12 @@ void orc_mesh_export(void* h, double* nodes, int32_t* en, int32_t* nb, double* n
   cp(val, m->val);
 }
 
+// Partner of each node (-1: none); all -1 for a mesh that is not periodic.
+void orc_mesh_pairs(void* h, int32_t* out) {
+  Mesh* m = (Mesh*)h;
+  for (int32_t i = 0; i < m->nn; ++i) out[i] = m->periodic ? m->pair[i] : -1;
+}
+
 void orc_elem_stiffness(void* h, int64_t e, double* K) {
   double k[4][4];
   elem_stiffness(*(Mesh*)h, e, k);
@@ -618,9 +665,9 @@ void orc_background(void* h, double f, double* bg) {
     for (int i = 0; i < 4; ++i) {
       double v = 0;
       for (int c = 0; c < 4; ++c) v += -f * m.wbv[e * 16 + i * 4 + c];
-      int32_t row = m.en[e * 4 + i];
-      if (m.bnd[row]) v = 0;
-      bg[row] += v;
+      int32_t node = m.en[e * 4 + i];
+      if (m.bnd[node]) v = 0;
+      bg[m.row_of(node)] += v;  // ES.cpp:247-251
     }
 }
 
@@ -736,7 +783,7 @@ void orc_deposit(void* h, int64_t n, const int32_t* elem, const double* w, const
       if (e < 0) continue;
       double val[4];
       particle_charge(m, e, w[p], species_q[typ[p]], val);
-      for (int i = 0; i < 4; ++i) a[m.en[(int64_t)e * 4 + i]] += to_fixed(val[i], S);
+      for (int i = 0; i < 4; ++i) a[m.row_of(m.en[(int64_t)e * 4 + i])] += to_fixed(val[i], S);  // PIC.cpp:132-135
     }
   }
   for (int64_t i = 0; i < m.nn; ++i) {
@@ -756,7 +803,7 @@ void orc_deposit_naive(void* h, int64_t n, const int32_t* elem, const double* w,
     if (e < 0) continue;
     double val[4];
     particle_charge(m, e, w[p], species_q[typ[p]], val);
-    for (int i = 0; i < 4; ++i) rhs[m.en[(int64_t)e * 4 + i]] += val[i];
+    for (int i = 0; i < 4; ++i) rhs[m.row_of(m.en[(int64_t)e * 4 + i])] += val[i];
   }
   for (int64_t i = 0; i < m.nn; ++i)
     if (m.bnd[i]) rhs[i] = 0.0;

@@ -39,7 +39,8 @@ def lib():
         P = C.c_void_p
         d, i32, i64, u64 = C.c_double, C.c_int, C.c_int64, C.c_uint64
         L.orc_mesh_create.restype = P
-        L.orc_mesh_create.argtypes = [i32, i32, i32, i32, P, P, d, u64, i32]
+        L.orc_mesh_create.argtypes = [i32, i32, i32, i32, P, P, d, u64, i32, i32]
+        L.orc_mesh_pairs.argtypes = [P, P]
         L.orc_mesh_destroy.argtypes = [P]
         L.orc_mesh_counts.argtypes = [P, P]
         L.orc_mesh_export.argtypes = [P] * 13
@@ -85,11 +86,13 @@ class Mesh:
     reference's node mapping 0-3 (mesh.cpp:763-792)."""
 
     def __init__(self, nx, ny, nz, split=6, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), jitter=0.0, jitter_seed=0,
-                 mapping=0):
+                 mapping=0, periodic=False):
         L = lib()
         self.lo = np.array(lo, dtype=np.float64)
         self.hi = np.array(hi, dtype=np.float64)
-        self.h = L.orc_mesh_create(nx, ny, nz, split, ptr(self.lo), ptr(self.hi), jitter, jitter_seed, mapping)
+        self.periodic = bool(periodic)
+        self.h = L.orc_mesh_create(nx, ny, nz, split, ptr(self.lo), ptr(self.hi), jitter, jitter_seed, mapping,
+                                   int(periodic))
         if not self.h:
             raise ValueError("bad mesh parameters or degenerate element")
         cnt = np.zeros(4, dtype=np.int64)
@@ -111,6 +114,8 @@ class Mesh:
         L.orc_mesh_export(self.h, *(ptr(a) for a in (self.nodes, self.elem_nodes, self.neighbors, self.normals,
                                                       self.jinv, self.detj, self.grad, self.wbv, self.boundary,
                                                       self.rowptr, self.col, self.val)))
+        self.pair = np.zeros(nn, np.int32)
+        L.orc_mesh_pairs(self.h, ptr(self.pair))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -40,8 +40,9 @@ def test_driver_built_and_usage():
 def test_driver_rejects_unknown_problem_and_bad_args():
     r = subprocess.run([DRIVER, "stray"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "unexpected argument" in r.stderr
-    r = subprocess.run([DRIVER, "--nx", "2", "--problem", "two-stream"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "two-stream" in r.stderr
+    # the two-stream problem's periodic mesh needs an even number of hexes in x (mesh.cpp:91-94)
+    r = subprocess.run([DRIVER, "--nx", "3", "--problem", "two-stream"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "even number of hexes" in r.stderr
 
 
 @pytest.mark.skipif(have_gpu_device(), reason="checks the refusal without a GPU")
