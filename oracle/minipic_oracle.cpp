@@ -26,6 +26,10 @@
 //     Tpetra serial sum (CG_solve.hpp:58,66,87,100).
 //   * Particle order: fill order is (fill call, element, ipart), one of the
 //     orders the reference's team append can produce (particle_list.hpp:103-139).
+//   * Periodic meshes (the two-stream problem): faces joined across x are
+//     stored as -2 - neighbour; pinned by geometry (the joined faces are the
+//     same triangle on opposite walls), the analytic periodic Poisson solution
+//     and one-period free streaming (tests/test_oracle.py).
 
 #include <cstdint>
 #include <cstring>

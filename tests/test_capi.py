@@ -43,6 +43,8 @@ CASES = [
     dict(nx=4, ny=4, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=1),
     dict(nx=4, ny=3, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=2),
     dict(nx=4, ny=4, nz=2, split=6, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, mapping=3),
+    dict(nx=6, ny=3, nz=4, split=5, lo=(-1, -1, -1), hi=(1, 1, 1), jitter=0.0, periodic=True),
+    dict(nx=4, ny=4, nz=3, split=6, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, periodic=True),
 ]
 
 
@@ -51,14 +53,19 @@ def test_host_setup_matches_oracle(mp, oracle, case):
     c = dict(case)
     jit = c.pop("jitter")
     mapping = c.pop("mapping", 0)
+    per = c.pop("periodic", False)
     o = oracle.Mesh(c["nx"], c["ny"], c["nz"], c["split"], lo=c["lo"], hi=c["hi"], jitter=jit, jitter_seed=11,
-                    mapping=mapping)
+                    mapping=mapping, periodic=per)
     p = mp.Mesh(c["nx"], c["ny"], c["nz"], c["split"], lo=c["lo"], hi=c["hi"], jitter=jit, jitter_seed=11,
-                background=1.75, mapping=mapping)
+                background=1.75, mapping=mapping, periodic=per)
     pairs = [("nodes", o.nodes), ("elem_nodes", o.elem_nodes), ("elem_nbrs", o.neighbors), ("normals", o.normals),
              ("inv_jacobian", o.jinv), ("det_jacobian", o.detj), ("grad_basis", o.grad), ("wbv", o.wbv),
              ("boundary", o.boundary), ("row_ptr", o.rowptr), ("col_idx", o.col), ("values", o.val),
              ("background", o.background(1.75))]
+    if per:
+        pairs.append(("node_pair", o.pair))
+    else:
+        assert p.array("node_pair") is None
     for name, ref in pairs:
         got = p.array(name)
         assert got.shape == ref.shape, name
@@ -66,6 +73,8 @@ def test_host_setup_matches_oracle(mp, oracle, case):
 
 
 def test_bad_arguments(mp):
+    with pytest.raises(mp.MpicError):
+        mp.Mesh(3, 2, 2, periodic=True)  # the periodic mesh needs an even nx (mesh.cpp:91-94)
     with pytest.raises(mp.MpicError):
         mp.Mesh(0, 1, 1)
     with pytest.raises(mp.MpicError):

@@ -155,3 +155,30 @@ def test_driver_loop_matches_oracle(oracle):
     for s in range(3):
         _, inf = O.step(om, p, phi, 0.1, bg=bg, S=S)
         assert abs(kes[s] - inf["ke"]) <= 1e-13 * abs(inf["ke"]), (s, kes[s], inf["ke"])
+
+
+@pytest.mark.gpu
+def test_driver_two_stream_matches_oracle(oracle):
+    """minipic_main --problem two-stream: the reference's periodic problem
+    (main.cpp:89-106) through the classes, against the oracle's steps."""
+    say("minipic_main: two-stream problem, periodic 5-tet 6x3x3, 4 steps")
+    r = subprocess.run([DRIVER, "--nx", "6", "--ny", "3", "--nz", "3", "--problem", "two-stream", "--nparts", "8",
+                        "--steps", "4", "--dt", "0.2", "--capacity-factor", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    kes = [float(l.split()[-1]) for l in r.stdout.splitlines() if l.startswith("Total KE is")]
+    assert len(kes) == 4
+    O = oracle
+    om = O.Mesh(6, 3, 3, 5, lo=(-1, -1, -1), hi=(1, 1, 1), periodic=True)
+    p = O.Particles(om.ne * 16)
+    p.fill(om, 4, 0, DENSITY / 2, vel=(0.5, 0, 0), temp=-0.05)
+    p.fill(om, 4, 0, DENSITY / 2, vel=(-0.5, 0, 0), temp=-0.05)
+    p.fill(om, 8, 2, DENSITY, vel=(0, 0, 0), temp=0.0)
+    S = O.charge_scale(om, p, None)
+    phi = np.zeros(om.nn)
+    for s in range(4):
+        rhs = O.deposit(om, p, None, S)
+        O.bicgstab(om, rhs, phi, tol=1e-6)
+        O.push(om, p, phi, 0.2)
+        ke = O.ke(p)
+        assert abs(kes[s] - ke) <= 1e-13 * abs(ke), (s, kes[s], ke)

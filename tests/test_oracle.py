@@ -18,6 +18,10 @@ Each oracle function and the check that pins it:
   walk                        -> free streaming with specular walls (unfolded
                                  closed form), containment, conservation
   KE                          -> direct numpy sum
+  periodic mesh (two-stream)  -> joined faces are the same (y, z) triangle on
+                                 opposite x walls, image rows, analytic
+                                 periodic Poisson solution (second order),
+                                 one-period free streaming, scipy BiCGStab
 """
 import json
 import math
@@ -351,3 +355,112 @@ def test_bicgstab_against_scipy(oracle, split):
     x3 = np.zeros(m.nn)
     s3, it3, r3 = oracle.bicgstab(m, b, x3, tol=1e-3)
     assert s3 == 0 and it3 < it1 and r3 < 1e-3
+
+
+# ---- periodic mesh (the two-stream problem), Mesh::Mesh with periodic = true ----
+
+def _faces_xyz(m, e, f):
+    FACE = [(0, 1, 3), (1, 2, 3), (0, 3, 2), (0, 2, 1)]
+    return m.nodes[m.elem_nodes[e, list(FACE[f])]]
+
+
+@pytest.mark.parametrize("split", [5, 6])
+def test_periodic_mesh_topology(oracle, split):
+    """x = -1 and x = +1 faces are joined (mesh.cpp:62-76,376-389); walls only in
+    y and z; Dirichlet nodes only on the y and z walls (mesh.cpp:410); image nodes
+    on x = +1 away from those walls pair with x = -1 (mesh.cpp:178-189)."""
+    nx, ny, nz = 6, 4, 5
+    m = oracle.Mesh(nx, ny, nz, split, lo=(-1, -1, -1), hi=(1, 1, 1), periodic=True)
+    nb = m.neighbors
+    assert (nb <= -2).sum() == 2 * 2 * ny * nz           # two triangles per quad, both sides
+    assert (nb == -1).sum() == 2 * 2 * (2 * nx * nz + 2 * nx * ny) // 2
+    x = m.nodes
+    on_yz = (np.abs(np.abs(x[:, 1]) - 1) < 1e-12) | (np.abs(np.abs(x[:, 2]) - 1) < 1e-12)
+    assert np.array_equal(m.boundary.astype(bool), on_yz)
+    images = np.where(m.pair >= 0)[0]
+    assert len(images) == (ny - 1) * (nz - 1)
+    assert np.allclose(x[images, 0], 1.0) and np.allclose(x[m.pair[images], 0], -1.0)
+    assert np.array_equal(x[images, 1:], x[m.pair[images], 1:])
+    for e in range(m.ne):
+        for f in range(4):
+            b = nb[e, f]
+            if b == -1:
+                continue
+            e2 = b if b >= 0 else -2 - b
+            back = np.where(nb[e2] == (e if b >= 0 else -2 - e))[0]
+            assert len(back) == 1
+            if b <= -2:  # the two faces are the same (y, z) triangle on opposite x walls
+                A, B = _faces_xyz(m, e, f), _faces_xyz(m, e2, back[0])
+                assert np.allclose(np.abs(A[:, 0]), 1.0) and np.allclose(A[:, 0] + B[0, 0], 0.0)
+                assert sorted(map(tuple, A[:, 1:])) == sorted(map(tuple, B[:, 1:]))
+
+
+def test_periodic_stiffness_and_poisson(oracle):
+    """Image rows are (-1 at the image, +1 at its partner) and no other row refers
+    to an image (ES.cpp:91-110,168-189); rows away from Dirichlet columns sum to
+    zero; the discrete solution converges to the x-periodic analytic one:
+    phi = cos(pi x) sin(pi (y+1)/2) sin(pi (z+1)/2), -lap phi = 1.5 pi^2 phi."""
+    import scipy.sparse.linalg as sla
+    errs = []
+    for n in (8, 16):
+        m = oracle.Mesh(n, n, n, 6, lo=(-1, -1, -1), hi=(1, 1, 1), periodic=True)
+        A = m.csr_scipy().tocsr()
+        images = np.where(m.pair >= 0)[0]
+        for g in images:
+            cols, vals = A.indices[A.indptr[g]:A.indptr[g + 1]], A.data[A.indptr[g]:A.indptr[g + 1]]
+            assert dict(zip(cols.tolist(), vals.tolist())) == {int(g): -1.0, int(m.pair[g]): 1.0}
+        other = np.setdiff1d(np.arange(m.nn), images)
+        assert A[other][:, images].nnz == 0
+        bnd = m.boundary.astype(bool)
+        rows = [r for r in other if not bnd[r] and not bnd[A.indices[A.indptr[r]:A.indptr[r + 1]]].any()]
+        assert np.abs(np.asarray(A[rows].sum(axis=1))).max() < 1e-13
+        # load vector of f = -1.5 pi^2 phi_exact at the 4 cubature points (the reference's
+        # convention K phi = -(f, N)), image contributions on the partner rows
+        a = 0.1381966011250105
+        cub = np.array([[a, a, a], [1 - 3 * a, a, a], [a, 1 - 3 * a, a], [a, a, 1 - 3 * a]])
+        ex = lambda p: np.cos(np.pi * p[..., 0]) * np.sin(np.pi * (p[..., 1] + 1) / 2) * np.sin(np.pi * (p[..., 2] + 1) / 2)
+        X = m.nodes[m.elem_nodes]                                   # [ne][4][3]
+        N = np.concatenate([1 - cub.sum(1, keepdims=True), cub], axis=1)  # [c][i]
+        P = np.einsum("ci,eid->ecd", N, X)
+        f = -1.5 * np.pi ** 2 * ex(P)                               # [ne][c]
+        contrib = -np.einsum("ec,eic->ei", f, m.wbv)
+        rows_of = np.where(m.pair >= 0, m.pair, np.arange(m.nn))[m.elem_nodes]
+        keep = ~bnd[m.elem_nodes]
+        b = np.zeros(m.nn)
+        np.add.at(b, rows_of[keep], contrib[keep])
+        phi = sla.spsolve(A.tocsc(), b)
+        assert np.abs(phi[images] - phi[m.pair[images]]).max() == 0.0
+        errs.append(np.abs(phi - ex(m.nodes)).max())
+    assert errs[1] < 0.35 * errs[0] and errs[1] < 0.02     # second order
+
+
+def test_periodic_free_streaming(oracle):
+    """Particles moving along x leave through x = +1 (or -1) and come back in on
+    the other side (particle_move.hpp:190-191): after one period of travel every
+    particle is back at its start, inside its element."""
+    m = oracle.Mesh(4, 3, 3, 5, lo=(-1, -1, -1), hi=(1, 1, 1), periodic=True)
+    p = oracle.Particles(m.ne * 3).fill(m, 3, 0, DENSITY, temp=0.0, vel=(0.5, 0.0, 0.0))
+    p.v[0, :p.n] *= np.where(np.arange(p.n) % 2 == 0, 1.0, -1.0)
+    x0 = p.x[:, :p.n].copy()
+    phi = np.zeros(m.nn)
+    crossings = 0
+    for _ in range(8):  # 8 * 0.5 * 0.5 = 2 = the period
+        st = oracle.push(m, p, phi, 0.5)
+        crossings += st["crossings"]
+        assert st["reflections"] == 0 and st["killed"] == 0
+    assert np.abs(p.x[:, :p.n] - x0).max() < 1e-12 and crossings > 0
+    assert _contained(m, p) > -1e-12
+
+
+def test_periodic_bicgstab_against_scipy(oracle):
+    """BiCGStab on the non-symmetric periodic stiffness matrix."""
+    import scipy.sparse.linalg as sla
+    m = oracle.Mesh(6, 4, 4, 5, lo=(-1, -1, -1), hi=(1, 1, 1), periodic=True)
+    A = m.csr_scipy()
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(m.nn)
+    b[m.boundary.astype(bool) | (m.pair >= 0)] = 0.0
+    xs = sla.spsolve(A.tocsc(), b)
+    x = np.zeros(m.nn)
+    s, it, rr = oracle.bicgstab(m, b, x, tol=1e-12)
+    assert s == 0 and np.abs(x - xs).max() <= 1e-9 * np.abs(xs).max()
