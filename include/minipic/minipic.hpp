@@ -368,8 +368,9 @@ class ES {
     minipic::detail::check(mpic_set_phi(c_->context(), p.data()), c_->ctx, "set_analytic_phi");
     c_->changed();
   }
-  // CG (CG_solve.hpp:54-114) from the current potential to |r| < tol |b|; false
-  // when it stops at max_iter.
+  // The configured solver (CG from the current potential, CG_solve.hpp:54-114,
+  // or BiCGStab from zero, :115-206) to |r| < tol |b|; false when it stops
+  // without converging.
   bool solve(FLOAT tol = 1.e-5) {
     int iters = 0;
     double relres = 0;
@@ -383,6 +384,31 @@ class ES {
   }
   int last_iterations() const { return last_iters_; }
   double last_relative_residual() const { return last_relres_; }
+  // The stiffness matrix, rhs and potential as MatrixMarket files (the writers
+  // of ES::solve, ES.cpp:224-228: matrix.mm, b.mm, x.mm), 1-based coordinates,
+  // 17 significant digits.
+  void write_matrix_market(const std::string& prefix = "") {
+    c_->sync_field(c_->rhs, c_->rhs_v, false);
+    c_->sync_field(c_->phi, c_->phi_v, true);
+    const mpic_mesh_desc& d = c_->desc;
+    std::FILE* f = std::fopen((prefix + "matrix.mm").c_str(), "w");
+    if (!f) throw minipic::Error(MPIC_ERR_ARG, "write_matrix_market: cannot open " + prefix + "matrix.mm");
+    std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%d %d %lld\n", d.num_nodes, d.num_nodes,
+                 (long long)d.nnz);
+    for (LO r = 0; r < d.num_nodes; ++r)
+      for (int32_t k = d.row_ptr[r]; k < d.row_ptr[r + 1]; ++k)
+        std::fprintf(f, "%d %d %.17g\n", r + 1, d.col_idx[k] + 1, d.values[k]);
+    std::fclose(f);
+    const std::vector<double>* vec[2] = {&c_->rhs, &c_->phi};
+    const char* name[2] = {"b.mm", "x.mm"};
+    for (int v = 0; v < 2; ++v) {
+      f = std::fopen((prefix + name[v]).c_str(), "w");
+      if (!f) throw minipic::Error(MPIC_ERR_ARG, "write_matrix_market: cannot open " + prefix + name[v]);
+      std::fprintf(f, "%%%%MatrixMarket matrix array real general\n%d 1\n", d.num_nodes);
+      for (double x : *vec[v]) std::fprintf(f, "%.17g\n", x);
+      std::fclose(f);
+    }
+  }
 
  private:
   static double call(const double* x, void* user) {

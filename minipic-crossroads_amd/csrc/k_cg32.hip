@@ -109,6 +109,7 @@ struct IRArgs {
   double *rho_hist, *alpha_hist;
   IRState* st;
   int it;
+  int spec;     // inner launch queued past the predicted end of the pass
   int nchunks;  // row chunks of BLOCK rows; partials are indexed by chunk
 };
 
@@ -239,6 +240,10 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
   if (chunk < 0) return;
   IRState* st = a.st;
+  // A launch queued past the predicted end of the pass first checks whether the
+  // pass has stopped (then it costs a state read instead of a full row load);
+  // the others issue their loads before that test.
+  if (a.spec && (st->done | st->in_done)) return;
   const int it = a.it;
   const int done = st->done | st->in_done;
   const double rho0_prev = st->in_rho0, tol2 = st->in_tol2;
@@ -371,6 +376,7 @@ IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
   a.alpha_hist = c.alpha_hist;
   a.st = c.st;
   a.it = it;
+  a.spec = 0;
   a.nchunks = c.nchunks;
   return a;
 }
@@ -390,8 +396,10 @@ hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bo
   return hipGetLastError();
 }
 
-hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s) {
-  hipLaunchKernelGGL(k_ir_cg, dim3(xcd_grid(c.nchunks)), dim3(BLOCK), 0, s, make_args(m, m32, c, it));
+hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, bool spec, hipStream_t s) {
+  IRArgs a = make_args(m, m32, c, it);
+  a.spec = spec ? 1 : 0;
+  hipLaunchKernelGGL(k_ir_cg, dim3(xcd_grid(c.nchunks)), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

@@ -350,7 +350,8 @@ struct IRBuffers {
 // on r/|r| unless converged.
 hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bool first, double tol, int max_iter,
                            double in_tol_min, hipStream_t s);
-hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, hipStream_t s);
+// spec: a launch past the predicted end of the pass (checks the stop first).
+hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, bool spec, hipStream_t s);
 // x += |r| d (FP64).
 hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, hipStream_t s);
 hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s);

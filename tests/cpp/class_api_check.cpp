@@ -7,6 +7,7 @@
 //   OUTDIR/stepped.txt    particles after two PIC::time_step calls
 //   OUTDIR/analytic.txt   phi of ES::solve on the load vector of sin(pi x) sin(pi y) sin(pi z)
 //   OUTDIR/efield.txt     ParticleList::E after set_analytic_phi(1 + 2x + 3y - z) + weight_Efield
+//   OUTDIR/matrix.mm, b.mm, x.mm  ES::write_matrix_market after that solve
 //   OUTDIR/info.txt       key value lines
 #include <algorithm>
 #include <cmath>
@@ -114,6 +115,7 @@ int main(int argc, char** argv) {
       es.set_analytic_RHS(source);
       const bool ok = es.solve(1e-12);
       info << "analytic_converged " << ok << "\n";
+      es.write_matrix_market(out + "/");  // matrix.mm, b.mm, x.mm
       Mesh mesh = data.mesh();
       {
         std::FILE* f = std::fopen((out + "/analytic.txt").c_str(), "w");

@@ -130,6 +130,14 @@ def test_class_interface_against_oracle(oracle, tmp_path):
     assert interior.size > 0 and np.all(rhs_a[interior] < 0)
     # the load vector of the constant 1 is the oracle's background of density 1
     assert abs(rhs_a.sum()) <= abs(om8.background(1.0).sum())
+    # ES::write_matrix_market: the oracle's stiffness matrix, and the rhs / potential just compared
+    import scipy.io
+    K = scipy.io.mmread(str(tmp_path / "matrix.mm")).tocsr()
+    K.sort_indices()
+    assert np.array_equal(K.indptr, om8.rowptr) and np.array_equal(K.indices, om8.col)
+    assert np.array_equal(K.data, om8.val)
+    assert np.array_equal(scipy.io.mmread(str(tmp_path / "b.mm")).ravel(), rhs_a)
+    assert np.array_equal(scipy.io.mmread(str(tmp_path / "x.mm")).ravel(), phi_a)
     # ES::set_analytic_phi with a linear potential: E = grad phi = (2, 3, -1)
     E = np.loadtxt(tmp_path / "efield.txt", ndmin=2)
     assert E.shape[0] == om8.ne * 4
