@@ -82,7 +82,7 @@ struct Row32 {
 
 __device__ __forceinline__ void stage_row(const DevMesh& m, const float* __restrict__ val, int row,
                                           const float4* __restrict__ q, Row32& R) {
-  const int slice = row >> 6, lane = row & 63;
+  const int slice = __builtin_amdgcn_readfirstlane(row >> 6), lane = row & 63;  // one slice per wave
   R.len = m.ell_len[slice];
   const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
   const float* __restrict__ vp = val + m.ell_off[slice] + lane;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
       w = fmaf(R.v[k], fmaf(-af, sk, R.g[k].x), w);
     }
     if (R.len > ROW_REGS) {
-      const int slice = i >> 6, lane = i & 63;
+      const int slice = __builtin_amdgcn_readfirstlane(i >> 6), lane = i & 63;  // one slice per wave
       const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
       const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
       for (int k = ROW_REGS; k < R.len; ++k) {
