@@ -37,6 +37,10 @@ struct mpic_batch {
   int *d_iters = nullptr, *d_conv = nullptr;
   double *d_relres = nullptr, *d_ke = nullptr;
   int64_t* d_count = nullptr;
+  int32_t* d_chunk_off = nullptr;           // [nprob+1] first statistics chunk of each problem
+  StatPartial* d_stat_part = nullptr;       // one partial per statistics chunk
+  int nchunks = 0;
+  int max_nodes = 0;                        // nodes of the largest problem
   double* phi_save = nullptr;               // tuner: potential before the trials
   unsigned long long* tune_acc = nullptr;   // tuner: scratch deposit
   int32_t* tune_count = nullptr;
@@ -120,6 +124,7 @@ BatchSolveArgs solve_args(mpic_batch* b) {
   a.tol2 = c->cfg.cg_tol * c->cfg.cg_tol;
   a.in_tol_min = 1e-5;
   a.max_iter = c->cfg.cg_max_iter;
+  a.max_nodes = b->max_nodes;
   return a;
 }
 
@@ -166,8 +171,8 @@ int enqueue_step(mpic_batch* b, double dt) {
   DevParticles live = c->P[c->cur];
   live.elem = c->P32[1 - c->cur32].elem;
   BCHK(b, launch_rank(c->m, live, c->n, true, true, c->sb, c->perm, s));
-  BCHK(b, launch_batch_stats(c->P32[1 - c->cur32].r, b->d_part_off, c->species, b->S_ke, b->d_ke, b->d_count,
-                             b->nprob, s));
+  BCHK(b, launch_batch_stats(c->P32[1 - c->cur32].r, b->d_part_off, b->d_chunk_off, b->nchunks, c->species,
+                             b->S_ke, b->d_stat_part, b->d_ke, b->d_count, b->nprob, s));
   return MPIC_OK;
 }
 
@@ -245,6 +250,7 @@ int mpic_batch_create(int nprob, mpic_mesh* const* meshes, const mpic_config* cf
   }
   b->node_off[nprob] = (int32_t)no;
   b->elem_off[nprob] = (int32_t)eo;
+  for (int p = 0; p < nprob; ++p) b->max_nodes = std::max(b->max_nodes, b->node_off[p + 1] - b->node_off[p]);
   mpic_mesh_desc u{};
   u.num_nodes = (int32_t)nn;
   u.num_elems = (int32_t)ne;
@@ -389,6 +395,14 @@ int mpic_batch_fill(mpic_batch* b, const mpic_batch_fill_desc* f) {
   c->acc64_deposited = true;
   if ((rc = rank_now(c, true, false, total))) return cfail(b, rc);
   BCHK(b, hipMemcpy(b->d_part_off, b->part_off.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice));
+  {
+    std::vector<int32_t> co(P + 1, 0);
+    for (int p = 0; p < P; ++p)
+      co[p + 1] = co[p] + (int32_t)((b->part_off[p + 1] - b->part_off[p] + STAT_CHUNK - 1) / STAT_CHUNK);
+    b->nchunks = co[P];
+    if ((rc = balloc(b, &b->d_chunk_off, P + 1)) || (rc = balloc(b, &b->d_stat_part, b->nchunks))) return rc;
+    BCHK(b, hipMemcpy(b->d_chunk_off, co.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice));
+  }
   // fixed-point scale of the kinetic-energy sums (the push's and the per-problem
   // ones): every term may grow to 256 times the fill's largest before a sum
   // could leave the 128-bit range

@@ -400,12 +400,28 @@ struct BatchSolveArgs {
   double* relres;           // [nprob] |r| / |b| at the end
   double tol2, in_tol_min;
   int max_iter;
+  int max_nodes;            // nodes of the largest problem
 };
+// LDS capacities (nodes) of the batch solve's inner residual: problems up to
+// BATCH_LDS_SMALL nodes (64 KB) or BATCH_LDS_LARGE (144 KB); larger ones keep
+// it in device memory.
+constexpr int BATCH_LDS_SMALL = 16384;
+constexpr int BATCH_LDS_LARGE = 36864;
 // Every problem's refinement solve, one workgroup of `threads` (256, 512 or 1024) per problem.
 hipError_t launch_batch_ir(const BatchSolveArgs& a, int nprob, int threads, hipStream_t s);
 // Per-problem kinetic energy (exact fixed-point sum at 2^S) and live count over
-// the particle ranges poff[p]..poff[p+1].
-hipError_t launch_batch_stats(const Rec32* r, const int64_t* poff, const double* species, int S, double* ke,
-                              int64_t* count, int nprob, hipStream_t s);
+// the particle ranges poff[p]..poff[p+1], cut into chunks of STAT_CHUNK
+// particles: chunk_off[p] is problem p's first chunk, chunk_off[nprob] ==
+// nchunks; part holds one partial per chunk.
+constexpr int STAT_CHUNK = 8192;
+struct StatPartial {
+  unsigned long long lo;
+  long long hi;
+  long long n;
+  long long pad;
+};
+hipError_t launch_batch_stats(const Rec32* r, const int64_t* poff, const int32_t* chunk_off, int nchunks,
+                              const double* species, int S, StatPartial* part, double* ke, int64_t* count, int nprob,
+                              hipStream_t s);
 
 }  // namespace mpic

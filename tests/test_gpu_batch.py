@@ -105,6 +105,35 @@ def test_batch_steps_match_single_problems(mp):
     b.close()
 
 
+def test_batch_solve_storage_paths(mp):
+    """The batch solve keeps the inner residual in LDS up to 16384 nodes (64 KB),
+    in a larger LDS tile up to 36864 nodes, in device memory beyond: sides 26
+    (19683 nodes) and 34 (42875 nodes) take the last two, each with a small
+    problem beside it, and match the same problems run alone."""
+    for sides in ((26, 3), (34, 2)):
+        say("batch solve, sides %s" % (sides,))
+        nppe = [2, 5]
+        meshes = [mp.Mesh(n, n, n, 6) for n in sides]
+        b = mp.Batch(meshes, mp.default_config())
+        b.fill([dict(nppe=nppe[k], temp=2 * 0.1 ** 2, density=DENSITY, pert_amp=0.05, seed_offset=k + 1)
+                for k in range(2)])
+        st = b.step(0.05)
+        assert st["all_converged"] == 1
+        ps = b.problem_stats()
+        for k, n in enumerate(sides):
+            assert ps["count"][k] == n ** 3 * 6 * nppe[k]
+            pm = mp.Mesh(n, n, n, 6, background=DENSITY)
+            ctx = mp.Context(pm.desc, mp.default_config(form=mp.FORM_MIXED))
+            ctx.reserve(pm.ne * nppe[k])
+            ctx.fill(nppe[k], 0, DENSITY, temp=2 * 0.1 ** 2, seed_offset=k + 1, pert_amp=0.05)
+            s1 = ctx.time_step(0.05)
+            phi_a, phi_b = ctx.phi(), b.phi(k)
+            assert np.linalg.norm(phi_a - phi_b) <= 1e-7 * np.linalg.norm(phi_a), (sides, k)
+            assert abs(ps["ke"][k] - s1["kinetic_energy"]) <= 1e-6 * s1["kinetic_energy"], (sides, k)
+            ctx.close()
+        b.close()
+
+
 def test_batch_graph_replays_equal_eager(mp):
     say("batch graph replays vs eager steps")
     ea = make_batch(mp)
