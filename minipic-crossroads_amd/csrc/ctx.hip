@@ -82,10 +82,20 @@ PushArgs push_args(mpic_ctx* c) {
 }
 
 // Reduce the per-workgroup partials of the last particle launch; synchronises.
+// Waits for the stream by polling it: a blocking wait adds tens of
+// microseconds of wake-up after the device has finished, on each of the few
+// waits of a time step.
+hipError_t wait_stream(hipStream_t s) {
+  hipError_t e;
+  while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+  }
+  return e;
+}
+
 int finish_partials(mpic_ctx* c, int parts, PushPartial* out) {
   HIPCHK(c, launch_push_finish(c->partial, parts, c->finish, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->finish_h, c->finish, sizeof(PushPartial), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, wait_stream(c->stream));
   *out = *c->finish_h;
   return MPIC_OK;
 }
@@ -177,22 +187,34 @@ int materialize(mpic_ctx* c) {
 // Counting sort of the current list as a permutation (applied by the next
 // fused push or by materialize), by element or by sort bucket.  n_live < 0: not
 // known, read back from the scan.
-int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live) {
+// The ranking's launches and the copy of its live count to start_h; the
+// ranking is pending once rank_finish has read that count.
+int rank_enqueue(mpic_ctx* c, bool by_bucket, bool counts_ready) {
   DevParticles live = c->P[c->cur];  // only the element array is read
   if (c->state32) live.elem = c->P32[c->cur32].elem;
   tstart(c, T_SORT);
   HIPCHK(c, launch_rank(c->m, live, c->n, by_bucket, counts_ready, c->sb, c->perm, c->stream));
   tstop(c, T_SORT);
-  if (n_live < 0) {
-    const int32_t nkeys = by_bucket ? c->m.nbuckets : c->m.ne;
-    HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + nkeys, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    n_live = *c->start_h;
-  }
-  c->perm_n = n_live;
+  const int32_t nkeys = by_bucket ? c->m.nbuckets : c->m.ne;
+  HIPCHK(c, hipMemcpyAsync(c->start_h, c->sb.start + nkeys, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  return MPIC_OK;
+}
+
+// After the stream has been waited for.
+void rank_finish(mpic_ctx* c, bool by_bucket) {
+  c->perm_n = *c->start_h;
   c->perm_pending = true;
   c->perm_by_bucket = by_bucket;
   c->sort_timed = true;
+}
+
+int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live) {
+  int rc;
+  if ((rc = rank_enqueue(c, by_bucket, counts_ready))) return rc;
+  HIPCHK(c, wait_stream(c->stream));
+  rank_finish(c, by_bucket);
+  if (n_live >= 0 && n_live != c->perm_n)
+    return fail(c, MPIC_ERR_STATE, "ranking: live count differs from the push's");
   return MPIC_OK;
 }
 
@@ -272,7 +294,7 @@ int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
     const int upto = std::min(max_iter + 1, it + batch);
     for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, c->cg, it, max_iter, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->cg_h, c->cg.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, wait_stream(c->stream));
     if (c->cg_h->done || it > max_iter) break;
     batch = 8;
   }
@@ -312,7 +334,7 @@ int do_bicgstab(mpic_ctx* c, double tol, int max_iter, int* iters, double* relre
     const int upto = std::min(max_iter + 1, it + 16);
     for (; it < upto; ++it) HIPCHK(c, launch_bi_iter(c->m, b, it, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->bi_h, b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, wait_stream(c->stream));
     if (c->bi_h->done || it > max_iter) break;
   }
   const BiState& s = *c->bi_h;
@@ -945,16 +967,22 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     c->n = c->perm_n;
     c->perm_pending = false;
   }
+  // the push's partials (over its grid) and, on a reordering step, the ranking
+  // by sort bucket from the push's counts, then one wait for both
   PushPartial r{};
   if (a.n > 0) {
-    if ((rc = finish_partials(c, partials_for(a.n), &r))) return rc;  // the push's grid
+    HIPCHK(c, launch_push_finish(c->partial, partials_for(a.n), c->finish, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->finish_h, c->finish, sizeof(PushPartial), hipMemcpyDeviceToHost, c->stream));
   }
   tstop(c, T_PUSH);
-  c->acc_deposited = true;
   if (c->sort_timed) {  // the previous step's ranking has completed
     c->ms[T_SORT] = telapsed(c, T_SORT);
     c->sort_timed = false;
   }
+  if (rank_after && (rc = rank_enqueue(c, true, true))) return rc;
+  HIPCHK(c, wait_stream(c->stream));
+  if (a.n > 0) r = *c->finish_h;
+  c->acc_deposited = true;
   double mx;
   std::memcpy(&mx, &r.ke_max_bits, 8);
   tstart(c, T_KE);
@@ -976,11 +1004,12 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   //    came from the push), also whenever particles were killed (cleanup_list)
   const int64_t n_live = c->n - r.killed;
   if (rank_after) {
-    if ((rc = rank_now(c, true, true, n_live))) return rc;
+    rank_finish(c, true);
+    if (c->perm_n != n_live) return fail(c, MPIC_ERR_STATE, "ranking: live count differs from the push's");
   } else if (r.killed > 0) {
     if ((rc = rank_now(c, true, false, n_live))) return rc;
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, wait_stream(c->stream));
   c->ms[T_DEPOSIT] = telapsed(c, T_DEPOSIT);
   c->ms[T_SOLVE] = telapsed(c, T_SOLVE);
   c->ms[T_PUSH] = telapsed(c, T_PUSH);

@@ -14,18 +14,19 @@ int invalidate_deposit(mpic_ctx* c);
 
 namespace {
 
-// Residual reduction of one FP32 inner pass before the FP64 residual replaces
-// the recursive one (reliable update; the direction is kept across passes).
-// At Kuhn 64^3 a warm-started solve to 1e-10 takes about the FP64 CG count
-// (262 vs 259 inner iterations in a CPU model) instead of 324 with restarts
-// at 1e-5.
-constexpr double IR_IN_TOL_MIN = 1e-2;
-// Inner launches queued past a pass's predicted length before the host polls,
-// and per poll after that.  Such a launch costs about 3 us when the pass has
-// already stopped (9 us without the early test) and 1-2 us more than a plain
-// launch when it has not.
-constexpr int IR_SPEC_TAIL = 4;
-constexpr int IR_SPEC_MORE = 8;
+// Inner iterations of one FP32 pass before the FP64 residual replaces the
+// recursive one (reliable update; the direction is kept across passes).  A
+// pass ends earlier only at the final tolerance (or IR_IN_TOL_MIN, far below
+// what FP32 reaches in IR_PASS_LEN iterations).  A CPU model at Kuhn 64^3
+// (warm start, tol 1e-10) takes 269 inner iterations for fixed passes of 30 to
+// 60 (296 at 80) against 259 for FP64 CG; a fixed length lets the launches of
+// several passes be queued without reading the device state in between.
+constexpr int IR_PASS_LEN = 50;
+constexpr double IR_IN_TOL_MIN = 1e-6;
+// Launches of the expected final pass past its expected stop + IR_SPEC_SLACK
+// first test whether the pass has stopped: about 3 us when it has (9 us
+// without the early test) and 1-2 us more than a plain launch when it has not.
+constexpr int IR_SPEC_SLACK = 2;
 
 int alloc_particles32(mpic_ctx* c) {
   if (c->cap == 0 || c->pmem32[0]) return MPIC_OK;
@@ -111,7 +112,7 @@ int mixed_state(mpic_ctx* c, bool to32) {
   if ((rc = invalidate_deposit(c))) return rc;
   if (c->cap > 0) {
     if (to32 && (rc = alloc_particles32(c))) return rc;
-    HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], 0, c->n, to32, c->stream));
+    HIPCHK(c, launch_convert(c->m, c->P[c->cur], c->P32[c->cur32], 0, c->n, to32, c->stream));
   }
   c->state32 = to32;
   c->ke_max_prev = 0;
@@ -119,14 +120,14 @@ int mixed_state(mpic_ctx* c, bool to32) {
 }
 
 int mixed_import(mpic_ctx* c, int64_t off, int64_t cnt) {
-  HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], off, cnt, true, c->stream));
+  HIPCHK(c, launch_convert(c->m, c->P[c->cur], c->P32[c->cur32], off, cnt, true, c->stream));
   return MPIC_OK;
 }
 
 int mixed_export(mpic_ctx* c) {
   int rc;
   if ((rc = materialize(c))) return rc;
-  HIPCHK(c, launch_convert(c->P[c->cur], c->P32[c->cur32], 0, c->n, false, c->stream));
+  HIPCHK(c, launch_convert(c->m, c->P[c->cur], c->P32[c->cur32], 0, c->n, false, c->stream));
   return MPIC_OK;
 }
 
@@ -153,9 +154,8 @@ int mixed_weight_charge(mpic_ctx* c) {
   return MPIC_OK;
 }
 
-// Iterative refinement: each pass launches the FP64 residual and test, then
-// batches of FP32 inner iterations sized from the same pass of the previous
-// solve, polling the device state once per batch.
+// Iterative refinement: passes of FP32 inner iterations between FP64
+// residuals, queued without reading the device state in between.
 int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   int rc;
   if ((rc = mixed_prepare(c))) return rc;
@@ -177,8 +177,9 @@ int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relre
       HIPCHK(c, launch_ir_setup(c->ir, tol, max_iter, c->stream));
       HIPCHK(c, launch_zero(c->sync_epoch, ((size_t)c->ir.nchunks + 2) / 2 * 8, c->stream));
       HIPCHK(c, launch_zero(c->sync_err, 8, c->stream));
-      const hipError_t e = launch_ir_persist(c->m, c->m32, c->ir, tol, max_iter, IR_IN_TOL_MIN, c->sync_epoch,
-                                             c->sync_err, c->stream);
+      const hipError_t e = launch_ir_persist(c->m, c->m32, c->ir, tol, max_iter, IR_IN_TOL_MIN,
+                                             std::min(IR_PASS_LEN, max_iter + 1), c->sync_epoch, c->sync_err,
+                                             c->stream);
       if (e == hipSuccess) {
         HIPCHK(c, hipMemcpyAsync(c->ir_h, c->ir.st, sizeof(IRState), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->sync_err_h, c->sync_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -197,40 +198,32 @@ int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relre
       c->persist_off = true;  // the per-iteration kernels from here on
     }
   }
-  std::vector<int> passes;
-  bool first = true;
-  for (int pass = 0;; ++pass) {
-    HIPCHK(c, launch_ir_outer(c->m, c->m32, c->ir, first, tol, max_iter, IR_IN_TOL_MIN, c->stream));
-    first = false;
-    const bool last_known = pass < (int)c->last_inner.size();
-    const bool expect_done = !c->last_inner.empty() && pass == (int)c->last_inner.size();
-    // the launches the same pass took last time (+1 for the stop test), then a
-    // few that first check whether the pass has stopped (cheap when it has)
-    int batch = last_known ? c->last_inner[pass] + 1 : (expect_done ? 0 : 16);
-    int spec = expect_done ? 0 : IR_SPEC_TAIL;
-    int it = 0;
-    bool solved = false;
-    while (true) {
-      const int upto = std::min(max_iter + 1, it + batch);
-      for (; it < upto; ++it) HIPCHK(c, launch_ir_inner(c->m, c->m32, c->ir, it, false, c->stream));
-      const int sp_upto = std::min(max_iter + 1, it + spec);
-      for (; it < sp_upto; ++it) HIPCHK(c, launch_ir_inner(c->m, c->m32, c->ir, it, true, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->ir_h, c->ir.st, sizeof(IRState), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      if (c->ir_h->done) {
-        solved = true;
-        break;
-      }
-      if (c->ir_h->in_done || it > max_iter) break;
-      batch = 0;
-      spec = IR_SPEC_MORE;
+  // The passes the previous solve took are queued at once (each pass: its
+  // inner launches, the update that ends it, the next outer test), then the
+  // state is read; after that one pass per read.  Launches after convergence
+  // return at once.
+  const int L = std::min(IR_PASS_LEN, max_iter + 1);
+  const int expect = c->solved_once ? (int)c->last_inner.size() : 1;
+  const int final_len = c->last_inner.empty() ? L : c->last_inner.back();
+  HIPCHK(c, launch_ir_outer(c->m, c->m32, c->ir, true, tol, max_iter, IR_IN_TOL_MIN, c->stream));
+  int pass = 0;
+  for (int round = 0;; ++round) {
+    const int np = round == 0 ? expect : 1;
+    for (int k = 0; k < np; ++k, ++pass) {
+      const bool final_pred = pass == expect - 1;
+      for (int it = 0; it < L; ++it)
+        HIPCHK(c, launch_ir_inner(c->m, c->m32, c->ir, it, final_pred && it > final_len + IR_SPEC_SLACK, c->stream));
+      HIPCHK(c, launch_ir_update(c->m, c->ir, L, c->stream));
+      HIPCHK(c, launch_ir_outer(c->m, c->m32, c->ir, false, tol, max_iter, IR_IN_TOL_MIN, c->stream));
     }
-    if (solved) break;
-    passes.push_back(c->ir_h->in_iters);
-    HIPCHK(c, launch_ir_update(c->m, c->ir, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ir_h, c->ir.st, sizeof(IRState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, wait_stream(c->stream));
+    if (c->ir_h->done) break;
   }
-  c->last_inner = passes;
   const IRState& s = *c->ir_h;
+  c->solved_once = true;
+  c->last_inner.assign(s.outer, L);
+  if (s.outer > 0) c->last_inner.back() = s.in_last;
   c->last_cg_iters = s.total_inner;
   if (iters) *iters = s.total_inner;
   if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
@@ -321,14 +314,22 @@ int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     c->n = c->perm_n;
     c->perm_pending = false;
   }
+  // the push's partials (over its grid) and, on a reordering step, the ranking
+  // by sort bucket from the push's counts, then one wait for both
   PushPartial r{};
-  if (a.n > 0 && (rc = finish_partials(c, push32_grid(a.n), &r))) return rc;  // the push's grid
+  if (a.n > 0) {
+    HIPCHK(c, launch_push_finish(c->partial, push32_grid(a.n), c->finish, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->finish_h, c->finish, sizeof(PushPartial), hipMemcpyDeviceToHost, c->stream));
+  }
   tstop(c, T_PUSH);
-  c->acc64_deposited = true;
   if (c->sort_timed) {
     c->ms[T_SORT] = telapsed(c, T_SORT);
     c->sort_timed = false;
   }
+  if (rank_after && (rc = rank_enqueue(c, true, true))) return rc;
+  HIPCHK(c, wait_stream(c->stream));
+  if (a.n > 0) r = *c->finish_h;
+  c->acc64_deposited = true;
   double mx;
   std::memcpy(&mx, &r.ke_max_bits, 8);
   tstart(c, T_KE);
@@ -345,14 +346,15 @@ int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   c->ke_max_prev = mx;
   st.move = mpic_move_stats{r.crossings, r.killed, r.capped, r.nonpos, r.reflections, r.relocated};
   c->steps++;
-  // 4. reordering by sort bucket (counts from the push), also after kills
+  // 4. reordering by sort bucket (ranked above), also after kills
   const int64_t n_live = c->n - r.killed;
   if (rank_after) {
-    if ((rc = rank_now(c, true, true, n_live))) return rc;
+    rank_finish(c, true);
+    if (c->perm_n != n_live) return fail(c, MPIC_ERR_STATE, "ranking: live count differs from the push's");
   } else if (r.killed > 0) {
     if ((rc = rank_now(c, true, false, n_live))) return rc;
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, wait_stream(c->stream));
   record_times(c);
   st.num_particles = c->perm_pending ? c->perm_n : c->n;
   st.charge_scale_bits = c->S64;

@@ -335,17 +335,21 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
 
 // x += |r| d in FP64; workgroup 0 then records what the next pass's start reads
 // (the pass count, this pass's scale, its last r.r, whether it may be
-// continued) and clears the inner state.
-__global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a) {
+// continued) and clears the inner state.  A pass that has not stopped by itself
+// ends here after pass_len iterations (the launches queued for it), as if its
+// stop test had passed at iteration pass_len.
+__global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a, int pass_len) {
   const int chunk = xcd_chunk(blockIdx.x, a.nchunks);
   IRState* st = a.st;
   if (chunk < 0 || st->done) return;
   const int i = chunk * BLOCK + threadIdx.x;
   if (i < a.m.nn) a.x[i] = fma(st->scale, (double)a.d[i], a.x[i]);
   if (chunk == 0 && threadIdx.x == 0) {
-    const int last = st->in_iters;
+    const int ended = st->in_done;
+    const int last = ended ? st->in_iters : pass_len;
+    if (!ended) st->total_inner += pass_len;
     st->outer = st->outer + 1;
-    st->cont_ok = st->in_ok && last > 0 ? 1 : 0;
+    st->cont_ok = (ended ? st->in_ok : 1) && last > 0 ? 1 : 0;
     st->scale_prev = st->scale;
     st->rho_last = last > 0 ? a.rho_hist[last - 1] : 0.0;
     st->in_last = last;
@@ -409,9 +413,9 @@ hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, hipStream_t s) {
+hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, int pass_len, hipStream_t s) {
   Mesh32 none{};
-  hipLaunchKernelGGL(k_ir_update, dim3(xcd_grid(c.nchunks)), dim3(BLOCK), 0, s, make_args(m, none, c, 0));
+  hipLaunchKernelGGL(k_ir_update, dim3(xcd_grid(c.nchunks)), dim3(BLOCK), 0, s, make_args(m, none, c, 0), pass_len);
   return hipGetLastError();
 }
 

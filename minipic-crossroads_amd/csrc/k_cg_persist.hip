@@ -239,6 +239,7 @@ struct PIRArgs {
   GridSync sync;
   double tol2, in_tol_min;
   int max_iter;
+  int pass_len;  // inner iterations after which a pass ends (k_ir_update's pass_len)
 };
 
 // The refinement solve of k_cg32.hip in one launch: per pass the FP64
@@ -367,6 +368,12 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_persist(PIRArgs a) {
         const double ps = reduce_partials(a.part_e + G, G);
         beta = gamma / rho_prev;
         denom = fma(beta * beta, ps, fma(2.0 * beta, rs, delta));  // same rounding in every build
+      }
+      if (it >= a.pass_len) {  // the pass ends at its length, as k_ir_update ends it
+        total_inner += it;
+        cont_ok = it > 0;
+        rho_last = it > 0 ? rho_prev : 0.0;
+        break;
       }
       const bool iconv = gamma == 0 || (it > 0 && gamma < in_tol2 * rho0);
       const bool istop = iconv || it >= a.max_iter || !(denom > 0);
@@ -508,7 +515,7 @@ hipError_t launch_cg_persist(const DevMesh& m, CGBuffers& c, double tol, int max
 namespace mpic {
 
 hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, double tol, int max_iter,
-                             double in_tol_min, unsigned* epoch, int* err, hipStream_t s) {
+                             double in_tol_min, int pass_len, unsigned* epoch, int* err, hipStream_t s) {
   if (!c.x || !c.b || !c.r || !c.rv[0] || !c.rv[1] || !c.wv[0] || !c.wv[1] || !c.sv[0] || !c.sv[1] || !c.part_a ||
       !c.part_b || !c.part_e || !c.st || !m32.ell_val || !epoch || !err || c.nchunks <= 0 || (int64_t)c.nchunks * BLOCK < m.nn)
     return hipErrorInvalidValue;
@@ -532,6 +539,7 @@ hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, 
   a.tol2 = tol * tol;
   a.in_tol_min = in_tol_min;
   a.max_iter = max_iter;
+  a.pass_len = pass_len;
   void* args[] = {&a};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_ir_persist), dim3(c.nchunks), dim3(BLOCK), args,
                                     0, s);

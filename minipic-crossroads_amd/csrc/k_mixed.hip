@@ -109,16 +109,22 @@ __device__ __forceinline__ bool near_face(const float R[3]) {
   return fminf(fminf(l0, R[0]), fminf(R[1], R[2])) < LOC_EPS;
 }
 
+// Rare events of the push, counted in LDS where they happen so that the walk
+// keeps a single counter (crossings) in registers.
+struct RareCount {
+  int capped, nonpos, reflections, relocated, killed;
+};
+
 // The tet walk of MoveParticle (particle_move.hpp:103-124,136-225) in FP32.
 // g holds the record of element e on entry and of the final element on return.
 __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict__ erec, int& e, ElemRec32& g,
                                        float X[3], float R[3], float V[3], float VR[3], float dt, int kill,
-                                       WalkCount& wc) {
+                                       int& crossings, RareCount& rare) {
   float tr = 1.0f;
   int iters = 0;
   while (tr > 0.0f) {
     if (iters++ >= WALK_CAP) {
-      wc.capped++;
+      atomicAdd(&rare.capped, 1);
       break;
     }
     const float ldt = dt * tr;
@@ -165,13 +171,13 @@ __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict
       const int e2 = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
       if (e2 >= 0) {
         e = e2;
-        wc.crossings++;
+        crossings++;
         g = load_erec32(erec, e);
         const float dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
         p2r_vecf(g.J, dl, R);
       } else {
         if (kill) return false;
-        wc.reflections++;
+        atomicAdd(&rare.reflections, 1);
         const double* nrm = m.normals + (size_t)e * 12 + face * 3;
         const float n0 = (float)nrm[0], n1 = (float)nrm[1], n2 = (float)nrm[2];
         const float vn = V[0] * n0 + V[1] * n1 + V[2] * n2;
@@ -181,7 +187,7 @@ __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict
       }
       p2r_vecf(g.J, V, VR);
     }
-    if (!(s > 0)) wc.nonpos++;
+    if (!(s > 0)) atomicAdd(&rare.nonpos, 1);
     tr -= s * tr;
   }
   return true;
@@ -263,14 +269,15 @@ __global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __re
   E[e] = make_float4((float)E0, (float)E1, (float)E2, 0.0f);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
+__global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
   __shared__ DepTable64 table;
+  __shared__ RareCount rare;
   dep64_clear(table);
+  if (threadIdx.x == 0) rare = RareCount{0, 0, 0, 0, 0};
   const bool gather = a.perm != nullptr;
-  WalkCount wc{};
-  long long killed = 0;
+  int crossings = 0;
   fx128 ke{0, 0};
-  unsigned long long kmax = 0;
+  unsigned kmax_hi = 0;  // high word of the largest KE term (terms are >= 0)
   int64_t lo, hi;
   block_range(a.n, &lo, &hi);
   __syncthreads();
@@ -290,13 +297,17 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           X[d] = rc.x[d];
-          R[d] = rc.xr[d];
           V[d] = rc.v[d];
         }
         t = (int)rc.type;
         w = rc.w;
         const float4 E = a.m32.E[e];
         ElemRec32 g = load_erec32(a.m32.erec, e);
+        {
+          // reference position of X in e (the record keeps no copy)
+          const float dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
+          p2r_vecf(g.J, dl, R);
+        }
         // Boris push with B = 0 (two half kicks), particle_move.hpp:364-392
         const float hq = a.dt * a.scale * (float)a.species[32 + t] * 0.5f;
         const float Ev[3] = {E.x, E.y, E.z};
@@ -307,23 +318,22 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
         }
         float VR[3];
         p2r_vecf(g.J, V, VR);
-        const bool alive = walk32(a.m, a.m32.erec, e, g, X, R, V, VR, a.dt, a.kill, wc);
+        const bool alive = walk32(a.m, a.m32.erec, e, g, X, R, V, VR, a.dt, a.kill, crossings, rare);
         if (alive && near_face(R)) {
           const Located L = locate64(a.m.geo, a.m.nbrs, e, X[0], X[1], X[2]);
           e = L.e;
           R[0] = L.r0;
           R[1] = L.r1;
           R[2] = L.r2;
-          wc.relocated += L.moves;
+          if (L.moves) atomicAdd(&rare.relocated, L.moves);
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           rc.x[d] = X[d];
-          rc.xr[d] = R[d];
           rc.v[d] = V[d];
         }
         enew = alive ? e : -1;
-        killed += alive ? 0 : 1;
+        if (!alive) atomicAdd(&rare.killed, 1);
         rc.elem = enew;
         store_rec(a.Q.r, p, rc);
         a.Q.elem[p] = enew;
@@ -337,12 +347,24 @@ __global__ __launch_bounds__(BLOCK) void k_push32(PushArgs32 a) {
       const double Vd[3] = {(double)V[0], (double)V[1], (double)V[2]};
       const double term = ke_term(w, a.species[16 + t], Vd);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
-      const unsigned long long tb = __double_as_longlong(term);
-      kmax = tb > kmax ? tb : kmax;
+      const unsigned tb = (unsigned)__double2hiint(term);
+      kmax_hi = tb > kmax_hi ? tb : kmax_hi;
     }
     if ((chunk + 1) % FLUSH_CHUNKS == 0) dep64_flush(table, a.acc, a.count, a.m.bucket);
   }
   dep64_flush(table, a.acc, a.count, a.m.bucket);
+  // dep64_flush ends with a barrier: every count is in
+  WalkCount wc{crossings, 0, 0, 0, 0};
+  long long killed = 0;
+  if (threadIdx.x == 0) {
+    wc.capped = rare.capped;
+    wc.nonpos = rare.nonpos;
+    wc.reflections = rare.reflections;
+    wc.relocated = rare.relocated;
+    killed = rare.killed;
+  }
+  // an upper bound of the largest term: its high word with the low word all ones
+  const unsigned long long kmax = kmax_hi ? ((unsigned long long)kmax_hi << 32) | 0xffffffffull : 0ull;
   block_partial(a.partial, ke, kmax, wc, killed);
 }
 
@@ -421,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_permute32(DevParticles32 src, DevPart
   dst.elem[q] = r.elem;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticles32 P32, int64_t off, int64_t n,
+__global__ __launch_bounds__(BLOCK) void k_convert(DevMesh m, DevParticles P64, DevParticles32 P32, int64_t off, int64_t n,
                                                    int to32) {
   const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (j >= n) return;
@@ -432,7 +454,6 @@ __global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticle
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       b.x[d] = (float)a.x[d];
-      b.xr[d] = (float)a.xr[d];
       b.v[d] = (float)a.v[d];
     }
     b.w = a.w;
@@ -448,8 +469,15 @@ __global__ __launch_bounds__(BLOCK) void k_convert(DevParticles P64, DevParticle
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       a.x[d] = (double)b.x[d];
-      a.xr[d] = (double)b.xr[d];
+      a.xr[d] = 0.0;
       a.v[d] = (double)b.v[d];
+    }
+    if (b.elem >= 0) {
+      // FP64 reference position of the (widened) FP32 position in its element
+      double g[12];
+      load12(m.geo, b.elem, g);
+      const double dl[3] = {a.x[0] - g[9], a.x[1] - g[10], a.x[2] - g[11]};
+      p2r_vec(g, dl, a.xr);
     }
     a.w = b.w;
     a.elem = b.elem;
@@ -575,10 +603,11 @@ hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst
   hipLaunchKernelGGL(k_permute32, dim3(grid_for(n)), dim3(BLOCK), 0, s, src, dst, perm, n);
   return hipGetLastError();
 }
-hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n, bool to32,
+hipError_t launch_convert(const DevMesh& m, const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n,
+                          bool to32,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_convert, dim3(grid_for(n)), dim3(BLOCK), 0, s, P64, P32, off, n, to32 ? 1 : 0);
+  hipLaunchKernelGGL(k_convert, dim3(grid_for(n)), dim3(BLOCK), 0, s, m, P64, P32, off, n, to32 ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s) {

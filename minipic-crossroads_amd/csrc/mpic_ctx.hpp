@@ -96,6 +96,7 @@ struct mpic_ctx {
   IRState* ir_h = nullptr;              // pinned
   int ir_hist_cap = 0;
   std::vector<int> last_inner;          // inner iterations per pass of the last solve
+  bool solved_once = false;             // last_inner holds a solve's passes
   unsigned long long* bad = nullptr;    // device counter
   unsigned long long* bad_h = nullptr;  // pinned
 
@@ -155,6 +156,9 @@ inline double telapsed(mpic_ctx* c, int t) {
 int finish_partials(mpic_ctx* c, int parts, PushPartial* out);
 int materialize(mpic_ctx* c);
 int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live);
+int rank_enqueue(mpic_ctx* c, bool by_bucket, bool counts_ready);
+void rank_finish(mpic_ctx* c, bool by_bucket);
+hipError_t wait_stream(hipStream_t s);
 // Buffers and capacities of the persistent solves (once per context).
 int persist_prepare(mpic_ctx* c);
 

@@ -71,31 +71,27 @@ __device__ __forceinline__ void store_rec(Rec64* __restrict__ base, int64_t i, c
 }
 __device__ __forceinline__ Rec32 load_rec(const Rec32* __restrict__ base, int64_t i) {
   const float4* q = reinterpret_cast<const float4*>(base + i);
-  const float4 a0 = q[0], a1 = q[1], a2 = q[2];
-  const double2 a3 = reinterpret_cast<const double2*>(base + i)[3];
+  const float4 a0 = q[0], a1 = q[1];
+  const int4 a2 = reinterpret_cast<const int4*>(base + i)[2];
   Rec32 r;
   r.x[0] = a0.x;
   r.x[1] = a0.y;
   r.x[2] = a0.z;
   r.elem = __float_as_int(a0.w);
-  r.xr[0] = a1.x;
-  r.xr[1] = a1.y;
-  r.xr[2] = a1.z;
-  r.id = (uint32_t)__float_as_int(a1.w);
-  r.v[0] = a2.x;
-  r.v[1] = a2.y;
-  r.v[2] = a2.z;
-  r.type = (uint32_t)__float_as_int(a2.w);
-  r.w = a3.x;
+  r.v[0] = a1.x;
+  r.v[1] = a1.y;
+  r.v[2] = a1.z;
+  r.type = (uint32_t)__float_as_int(a1.w);
+  r.w = __hiloint2double(a2.y, a2.x);
+  r.id = (uint32_t)a2.z;
   r.pad = 0;
   return r;
 }
 __device__ __forceinline__ void store_rec(Rec32* __restrict__ base, int64_t i, const Rec32& r) {
   float4* q = reinterpret_cast<float4*>(base + i);
   q[0] = float4{r.x[0], r.x[1], r.x[2], __int_as_float(r.elem)};
-  q[1] = float4{r.xr[0], r.xr[1], r.xr[2], __int_as_float((int)r.id)};
-  q[2] = float4{r.v[0], r.v[1], r.v[2], __int_as_float((int)r.type)};
-  reinterpret_cast<double2*>(base + i)[3] = double2{r.w, 0.0};
+  q[1] = float4{r.v[0], r.v[1], r.v[2], __int_as_float((int)r.type)};
+  reinterpret_cast<int4*>(base + i)[2] = int4{__double2loint(r.w), __double2hiint(r.w), (int)r.id, 0};
 }
 
 // physToReferenceVector, mesh.hpp:245-250: out_r = ((0 + J_r0 v0) + J_r1 v1) + J_r2 v2.

@@ -90,19 +90,20 @@ struct DevParticles {
   int64_t cap;
 };
 
-// Precision-form particle record, 64 B: FP32 position, reference position and
-// velocity; the weight stays FP64 because charges are summed exactly from it.
+// Precision-form particle record, 48 B: FP32 position and velocity; the weight
+// stays FP64 because charges are summed exactly from it.  The reference
+// position is not stored: the push derives it from the position and the
+// element's inverse Jacobian, and the FP64 export computes it in FP64.
 struct alignas(16) Rec32 {
   float x[3];
   int32_t elem;
-  float xr[3];
-  uint32_t id;
   float v[3];
   uint32_t type;
   double w;
-  double pad;
+  uint32_t id;
+  uint32_t pad;
 };
-static_assert(sizeof(Rec32) == 64, "precision-form particle record is four 16-B words");
+static_assert(sizeof(Rec32) == 48, "precision-form particle record is three 16-B words");
 
 struct DevParticles32 {
   Rec32* r;       // [cap]
@@ -308,9 +309,10 @@ hipError_t launch_ke32(const DevParticles32& P, int64_t n, const double* species
                        PushPartial* partial, hipStream_t s);
 hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst, const int32_t* perm, int64_t n,
                             hipStream_t s);
-// FP64 <-> precision-form copies of particles [off, off+n) (to32: P64 -> P32).
-hipError_t launch_convert(const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n, bool to32,
-                          hipStream_t s);
+// FP64 <-> precision-form copies of particles [off, off+n) (to32: P64 -> P32);
+// the FP64 reference position is computed from the position in the element.
+hipError_t launch_convert(const DevMesh& m, const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n,
+                          bool to32, hipStream_t s);
 hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s);  // FP32 walk records
 // Ownership check: adds to *bad the live particles whose element does not
@@ -352,13 +354,14 @@ hipError_t launch_ir_outer(const DevMesh& m, const Mesh32& m32, IRBuffers& c, bo
                            double in_tol_min, hipStream_t s);
 // spec: a launch past the predicted end of the pass (checks the stop first).
 hipError_t launch_ir_inner(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it, bool spec, hipStream_t s);
-// x += |r| d (FP64).
-hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, hipStream_t s);
+// x += |r| d (FP64); ends a pass that has not stopped by itself after pass_len
+// inner iterations.
+hipError_t launch_ir_update(const DevMesh& m, IRBuffers& c, int pass_len, hipStream_t s);
 hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s);
 // The whole precision-form refinement solve as one cooperative launch; same
 // results as launch_ir_outer / launch_ir_inner / launch_ir_update.
 hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, double tol, int max_iter,
-                             double in_tol_min, unsigned* epoch, int* err, hipStream_t s);
+                             double in_tol_min, int pass_len, unsigned* epoch, int* err, hipStream_t s);
 
 struct SortBuffers {
   int32_t* count;    // [ne+1]
