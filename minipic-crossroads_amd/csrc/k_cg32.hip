@@ -42,10 +42,14 @@ __device__ __forceinline__ double block_sum(double v) {
   return (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
+__device__ __forceinline__ double uniform(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double reduce(const double* __restrict__ part, int P) {
-  double acc = 0;
-  for (int i = threadIdx.x; i < P; i += BLOCK) acc += part[i];
-  return block_sum(acc);
+  return block_sum(thread_partials(part, P));
 }
 
 // FP64 row of A x (sliced ELL, entries in CSR order).
@@ -80,14 +84,18 @@ struct Row32 {
   float3 g[ROW_REGS];  // (r, w, s) at column c
 };
 
-__device__ __forceinline__ void stage_row(const DevMesh& m, const float* __restrict__ val, int row,
-                                          const float4* __restrict__ q, Row32& R) {
+__device__ __forceinline__ void stage_cols(const DevMesh& m, int row, Row32& R) {
   const int slice = __builtin_amdgcn_readfirstlane(row >> 6), lane = row & 63;  // one slice per wave
   R.len = m.ell_len[slice];
   const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
-  const float* __restrict__ vp = val + m.ell_off[slice] + lane;
 #pragma unroll
   for (int k = 0; k < ROW_REGS; ++k) R.c[k] = k < R.len ? cp[k * 64] : row;
+}
+
+__device__ __forceinline__ void stage_gathers(const DevMesh& m, const float* __restrict__ val, int row,
+                                              const float4* __restrict__ q, Row32& R) {
+  const int slice = __builtin_amdgcn_readfirstlane(row >> 6), lane = row & 63;
+  const float* __restrict__ vp = val + m.ell_off[slice] + lane;
 #pragma unroll
   for (int k = 0; k < ROW_REGS; ++k) {
     R.v[k] = k < R.len ? vp[k * 64] : 0.0f;
@@ -250,26 +258,26 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   const int max_iter = st->max_iter;
   const bool cont0 = it == 0 && st->cont;
   const bool carry = it > 0 || cont0;  // beta != 0: the update uses p and s
-  const double rho_prev = it > 0 ? a.rho_hist[it - 1] : st->rho_c;
-  const double alpha_prev = it > 0 ? a.alpha_hist[it - 1] : 1.0;
+  // unconditional loads (see thread_partials), selected after
+  const double rho_h = a.rho_hist[it > 0 ? it - 1 : 0], alpha_h = a.alpha_hist[it > 0 ? it - 1 : 0];
+  const double rho_prev = it > 0 ? rho_h : st->rho_c;
+  const double alpha_prev = it > 0 ? alpha_h : 1.0;
   const int i = chunk * BLOCK + threadIdx.x;
   const bool row = i < a.m.nn;
-  double pg = 0, pd = 0;
-  for (int k = threadIdx.x; k < a.nchunks; k += BLOCK) {
-    pg += a.part_cur[k];
-    pd += a.part_cur[a.nchunks + k];
-  }
+  // loads in flight together: the previous launch's partials and the row's
+  // columns; the partials are added while the gathers are issued
+  double va[PART_BATCH], vb[PART_BATCH];
+  partial_loads2(a.part_cur, a.part_cur + a.nchunks, a.nchunks, va, vb);
   Row32 R;
-  float4 own = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float di = 0;
-  if (row) {
-    stage_row(a.m, a.val32, i, a.q_cur, R);
-    own = a.q_cur[i];
-    di = a.d[i];
-  }
+  if (row) stage_cols(a.m, i, R);
+  double pg, pd;
+  partial_sums2(a.part_cur, a.part_cur + a.nchunks, a.nchunks, va, vb, pg, pd);
+  if (row) stage_gathers(a.m, a.val32, i, a.q_cur, R);
   if (done) return;
-  const double gamma = block_sum(pg);
-  const double delta = block_sum(pd);
+  // the sums are equal in every lane: scalar registers for them and what
+  // follows from them leave the vector registers to the staged row
+  const double gamma = uniform(block_sum(pg));
+  const double delta = uniform(block_sum(pd));
   const double rho0 = it == 0 ? gamma : rho0_prev;
   double beta = 0, alpha = 0, denom = delta;
   if (it > 0) {
@@ -300,6 +308,10 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   const float bf = (float)beta, af = (float)alpha;
   double tg = 0, td = 0;
   if (row) {
+    // the row's own word (one of the gathered ones: the load hits the cache)
+    // and d, read late so that they hold no registers while the row is staged
+    const float4 own = a.q_cur[i];
+    const float di = a.d[i];
     const float ri = own.x, wi = own.y;
     const float p = carry ? fmaf(bf, own.w, ri) : ri;
     const float s = carry ? fmaf(bf, own.z, wi) : wi;

@@ -78,8 +78,7 @@ __device__ __forceinline__ double block_tree(double v) {
 }
 
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int P) {
-  double acc = 0;
-  for (int i = threadIdx.x; i < P; i += BLOCK) acc += part[i];
+  const double acc = thread_partials(part, P);
   return block_tree(acc);
 }
 

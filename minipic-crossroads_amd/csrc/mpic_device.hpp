@@ -18,6 +18,53 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
   return c < n ? c : -1;
 }
 
+// A thread's share of P chunk partials, part[t] + part[t + BLOCK] + ... in that
+// order (t = threadIdx.x), with the first PART_BATCH loads issued together: a
+// plain loop waits for each load before the next (one memory round trip per
+// BLOCK partials, five at a 65^3-node mesh).  The loads are unconditional (at
+// a clamped index, P > 0): a conditional load's register would have to be
+// waited for where the paths join.
+constexpr int PART_BATCH = 5;
+__device__ __forceinline__ double thread_partials(const double* __restrict__ part, int P) {
+  double v[PART_BATCH];
+#pragma unroll
+  for (int u = 0; u < PART_BATCH; ++u) v[u] = part[min((int)threadIdx.x + u * BLOCK, P - 1)];
+  double acc = 0;
+#pragma unroll
+  for (int u = 0; u < PART_BATCH; ++u)
+    if (threadIdx.x + u * BLOCK < P) acc += v[u];
+  for (int i = threadIdx.x + PART_BATCH * BLOCK; i < P; i += BLOCK) acc += part[i];
+  return acc;
+}
+// The same for two partial arrays read together, in two steps so that other
+// loads can be issued in between: partial_loads2 issues the batch,
+// partial_sums2 adds it (and any partials past it).
+__device__ __forceinline__ void partial_loads2(const double* __restrict__ pa, const double* __restrict__ pb, int P,
+                                               double (&va)[PART_BATCH], double (&vb)[PART_BATCH]) {
+#pragma unroll
+  for (int u = 0; u < PART_BATCH; ++u) {
+    const int i = min((int)threadIdx.x + u * BLOCK, P - 1);
+    va[u] = pa[i];
+    vb[u] = pb[i];
+  }
+}
+__device__ __forceinline__ void partial_sums2(const double* __restrict__ pa, const double* __restrict__ pb, int P,
+                                              const double (&va)[PART_BATCH], const double (&vb)[PART_BATCH],
+                                              double& a, double& b) {
+  a = 0;
+  b = 0;
+#pragma unroll
+  for (int u = 0; u < PART_BATCH; ++u)
+    if (threadIdx.x + u * BLOCK < P) {
+      a += va[u];
+      b += vb[u];
+    }
+  for (int i = threadIdx.x + PART_BATCH * BLOCK; i < P; i += BLOCK) {
+    a += pa[i];
+    b += pb[i];
+  }
+}
+
 // 12 / 16 consecutive doubles of element e (records are 96 / 128 B, 16-B aligned).
 __device__ __forceinline__ void load12(const double* __restrict__ base, int e, double o[12]) {
   const double2* q = reinterpret_cast<const double2*>(base + (size_t)e * 12);
