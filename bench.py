@@ -214,12 +214,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=10, help="timed steps of the ensemble (batching form); 0 = skip")
+    ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run two 2 GiB device-to-device copies whose FETCH_SIZE/WRITE_SIZE calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
+    if args.ensemble_only:
+        print(json.dumps(run_ensemble(args)), flush=True)
+        return
     import numpy as np
     gm = run_gpu(args, 1)
     gd = run_gpu(args, 0)
