@@ -16,7 +16,7 @@
 // launch_permute applies it).  Reading in storage order and writing each
 // record to its slot instead was measured slower on MI355X (the scattered
 // 64/96-B record writes cost more than the scattered reads).  As in the reference, the order inside one key is
-// not specified (it follows the order of the per-wavefront atomics); the
+// not specified (it follows the order of the ranking's atomics); the
 // identity key (id) travels with the particle.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
@@ -121,19 +121,59 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
   }
 }
 
+// Ranking, one workgroup per RANK_ITEMS * BLOCK consecutive particles: each
+// particle takes a local rank among the workgroup's particles with its key from
+// an LDS table (key, count), then one thread per key the workgroup holds adds
+// the key's count to cursor[key] with one atomic, and every particle's slot is
+// the returned base plus its local rank.  Between reorderings a particle moves
+// only to neighbouring elements, so a workgroup's 1024 particles hold a few
+// dozen keys: a few dozen global atomics per workgroup instead of one per key
+// per wavefront (about four times as many).
+constexpr int RANK_ITEMS = 4;
+constexpr int RANK_TABLE = 2 * RANK_ITEMS * BLOCK;  // LDS slots: at most half used
+constexpr int RANK_BITS = 11;
+static_assert(RANK_TABLE == 1 << RANK_BITS, "rank table size");
+
 __global__ __launch_bounds__(BLOCK) void k_rank(const int32_t* __restrict__ elem, const int32_t* __restrict__ kmap,
                                                 int64_t n, int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
-  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int e = sort_key(elem, kmap, p, n);
-  unsigned long long grp;
-  const int leader = wave_group_leader(e, &grp);
-  int base = 0;
-  if (leader == lane) base = atomicAdd(cursor + e, (int)__popcll(grp));
-  base = __shfl(base, leader < 0 ? lane : leader);
-  if (e < 0) return;
-  const unsigned long long below = (lane == 0) ? 0ull : (grp & ((1ull << lane) - 1ull));
-  perm[base + __popcll(below)] = (int32_t)p;
+  __shared__ int tkey[RANK_TABLE];
+  __shared__ int tcnt[RANK_TABLE];
+  for (int k = threadIdx.x; k < RANK_TABLE; k += BLOCK) {
+    tkey[k] = -1;
+    tcnt[k] = 0;
+  }
+  __syncthreads();
+  const int64_t p0 = (int64_t)blockIdx.x * (RANK_ITEMS * BLOCK) + threadIdx.x;
+  int slot[RANK_ITEMS], rank[RANK_ITEMS];
+#pragma unroll
+  for (int u = 0; u < RANK_ITEMS; ++u) {
+    const int e = sort_key(elem, kmap, p0 + u * BLOCK, n);
+    slot[u] = -1;
+    rank[u] = 0;
+    if (e < 0) continue;
+    // linear probing; the table never fills (RANK_TABLE > keys of a workgroup)
+    unsigned h = ((unsigned)e * 2654435761u) >> (32 - RANK_BITS);
+    while (true) {
+      int k = __hip_atomic_load(&tkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (k == -1) {
+        const int old = atomicCAS(&tkey[h], -1, e);
+        k = old == -1 ? e : old;
+      }
+      if (k == e) break;
+      h = (h + 1) & (RANK_TABLE - 1);
+    }
+    slot[u] = (int)h;
+    rank[u] = atomicAdd(&tcnt[h], 1);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < RANK_TABLE; k += BLOCK) {
+    const int key = tkey[k];
+    if (key >= 0) tcnt[k] = atomicAdd(cursor + key, tcnt[k]);  // the key's base slot
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RANK_ITEMS; ++u)
+    if (slot[u] >= 0) perm[tcnt[slot[u]] + rank[u]] = (int32_t)(p0 + u * BLOCK);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_permute(DevParticles src, DevParticles dst, const int32_t* __restrict__ perm,
@@ -163,7 +203,8 @@ hipError_t launch_rank(const DevMesh& m, const DevParticles& P, int64_t n, bool 
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, sb.blocksum, ntiles);
   hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLOCK), 0, s, sb.count, nk1, sb.blocksum, sb.start, sb.cursor,
                      nkeys);
-  if (n > 0) hipLaunchKernelGGL(k_rank, dim3(gp), dim3(BLOCK), 0, s, P.elem, kmap, n, sb.cursor, perm);
+  const unsigned gr = (unsigned)((n + RANK_ITEMS * BLOCK - 1) / (RANK_ITEMS * BLOCK));
+  if (n > 0) hipLaunchKernelGGL(k_rank, dim3(gr), dim3(BLOCK), 0, s, P.elem, kmap, n, sb.cursor, perm);
   return hipGetLastError();
 }
 
