@@ -33,6 +33,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Two workgroup sums with one pair of barriers; each is the same tree as block_sum.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+  __shared__ double ws[2][BLOCK / 64];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    ws[0][threadIdx.x >> 6] = a;
+    ws[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  a = (ws[0][0] + ws[0][1]) + (ws[0][2] + ws[0][3]);
+  b = (ws[1][0] + ws[1][1]) + (ws[1][2] + ws[1][3]);
+}
+
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double ws[BLOCK / 64];
   v = wave_sum(v);
@@ -141,8 +156,8 @@ __global__ __launch_bounds__(BLOCK) void k_ir_resid(IRArgs a) {
     tb = bi * bi;
     tr = ri * ri;
   }
-  const double sb = block_sum(tb);
-  const double sr = block_sum(tr);
+  double sb = tb, sr = tr;
+  block_sum2(sb, sr);
   if (threadIdx.x == 0) {
     a.part_bb[chunk] = sb;
     a.part_rr[chunk] = sr;
@@ -215,15 +230,15 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
     tg = (double)ri * (double)ri;
     td = (double)w * (double)ri;
   }
-  const double sg = block_sum(tg);
-  const double sd = block_sum(td);
+  double sg = tg, sd = td;
+  block_sum2(sg, sd);
   if (threadIdx.x == 0) {
     a.part_nxt[chunk] = sg;
     a.part_nxt[a.nchunks + chunk] = sd;
   }
   if (cont) {
-    const double se = block_sum(te);
-    const double sf = block_sum(tf);
+    double se = te, sf = tf;
+    block_sum2(se, sf);
     if (threadIdx.x == 0) {
       a.part_e[chunk] = se;
       a.part_e[a.nchunks + chunk] = sf;
@@ -337,8 +352,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
     tg = (double)rn * (double)rn;
     td = (double)w * (double)rn;
   }
-  const double sg = block_sum(tg);
-  const double sd = block_sum(td);
+  double sg = tg, sd = td;
+  block_sum2(sg, sd);
   if (threadIdx.x == 0) {
     a.part_nxt[chunk] = sg;
     a.part_nxt[a.nchunks + chunk] = sd;
