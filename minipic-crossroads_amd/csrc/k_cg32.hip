@@ -68,13 +68,26 @@ __device__ __forceinline__ double reduce(const double* __restrict__ part, int P)
 }
 
 // FP64 row of A x (sliced ELL, entries in CSR order).
+// Loads of a row's first STAGE entries issued together; later entries in a loop.
+constexpr int STAGE = 15;
+
+// FP64 row of A x: acc = 0; acc += val_k * x(col_k) in stored order.
 __device__ __forceinline__ double row64(const DevMesh& m, int row, const double* __restrict__ x) {
   const int slice = row >> 6, lane = row & 63;
   const int len = m.ell_len[slice];
   const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
   const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
+  int c[STAGE];
+#pragma unroll
+  for (int k = 0; k < STAGE; ++k) c[k] = k < len ? cp[k * 64] : row;
   double acc = 0;
-  for (int k = 0; k < len; ++k) acc += vp[k * 64] * x[cp[k * 64]];
+#pragma unroll
+  for (int k = 0; k < STAGE; ++k) {
+    const double v = k < len ? vp[k * 64] : 0.0;
+    const double xv = x[c[k]];
+    if (k < len) acc += v * xv;
+  }
+  for (int k = STAGE; k < len; ++k) acc += vp[k * 64] * x[cp[k * 64]];
   return acc;
 }
 
@@ -211,8 +224,17 @@ __global__ __launch_bounds__(BLOCK) void k_ir_start(IRArgs a, double in_tol_min)
     const int len = a.m.ell_len[slice];
     const int* __restrict__ cp = a.m.ell_col + a.m.ell_off[slice] + lane;
     const float* __restrict__ vp = a.val32 + a.m.ell_off[slice] + lane;
+    int c[STAGE];
+#pragma unroll
+    for (int k = 0; k < STAGE; ++k) c[k] = k < len ? cp[k * 64] : i;
     float w = 0;
-    for (int k = 0; k < len; ++k) w = fmaf(vp[k * 64], (float)(a.r[cp[k * 64]] * inv), w);
+#pragma unroll
+    for (int k = 0; k < STAGE; ++k) {
+      const float v = k < len ? vp[k * 64] : 0.0f;
+      const double rk = a.r[c[k]];
+      if (k < len) w = fmaf(v, (float)(rk * inv), w);
+    }
+    for (int k = STAGE; k < len; ++k) w = fmaf(vp[k * 64], (float)(a.r[cp[k * 64]] * inv), w);
     const float ri = (float)(a.r[i] * inv);
     float sc = 0.0f, pc = 0.0f;
     if (cont) {
