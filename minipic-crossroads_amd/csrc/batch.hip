@@ -107,7 +107,7 @@ int ensure_ranked(mpic_batch* b) {
 BatchSolveArgs solve_args(mpic_batch* b) {
   mpic_ctx* c = b->c;
   BatchSolveArgs a;
-  a.m = c->m;
+  a.m = diag_first(c->m, c->m32);
   a.val32 = c->m32.ell_val;
   a.node_off = b->d_node_off;
   a.x = c->phi;

@@ -63,9 +63,12 @@ int mixed_prepare(mpic_ctx* c) {
     const int32_t nn = c->m.nn, ne = c->m.ne;
     float* erec32;
     float* ell32;
+    int32_t* col32;
+    double* val64p;
     float4* E32;
     int rc;
     if ((rc = dalloc(c, &erec32, (size_t)ne * 16)) || (rc = dalloc(c, &ell32, (size_t)c->ell_entries)) ||
+        (rc = dalloc(c, &col32, (size_t)c->ell_entries)) || (rc = dalloc(c, &val64p, (size_t)c->ell_entries)) ||
         (rc = dalloc(c, &E32, (size_t)ne)) || (rc = dalloc(c, &c->acc64, (size_t)ne)) ||
         (rc = dalloc(c, &c->ir.r, nn)) || (rc = dalloc(c, &c->ir.d, nn)) || (rc = dalloc(c, &c->ir.p, nn)) ||
         (rc = dalloc(c, &c->ir.rv[0], nn)) || (rc = dalloc(c, &c->ir.rv[1], nn)) ||
@@ -79,17 +82,24 @@ int mixed_prepare(mpic_ctx* c) {
       return rc;
     if (!c->background.empty() && (rc = dupload(c, &c->bg64, c->background.data(), c->background.size()))) return rc;
     HIPCHK(c, launch_erec32(c->m, erec32, c->stream));
-    HIPCHK(c, launch_to_float(c->m.ell_val, ell32, (size_t)c->ell_entries, c->stream));
+    HIPCHK(c, launch_zero(c->bad, 8, c->stream));
+    HIPCHK(c, launch_ell_diag_first(c->m, col32, val64p, ell32, reinterpret_cast<int*>(c->bad), c->stream));
     HIPCHK(c, launch_zero(c->acc64, (size_t)ne * 8, c->stream));
     HIPCHK(c, launch_zero(c->ir.st, sizeof(IRState), c->stream));
     if (!c->ir_h && hipHostMalloc((void**)&c->ir_h, sizeof(IRState)) != hipSuccess)
       return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
     if (!c->bad_h && hipHostMalloc((void**)&c->bad_h, sizeof(unsigned long long)) != hipSuccess)
       return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->bad_h, c->bad, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, wait_stream(c->stream));
+    if (*c->bad_h != 0)
+      return fail(c, MPIC_ERR_STATE, "the precision form needs a diagonal entry in every row of the stiffness matrix");
     c->ir.nchunks = c->cg.nchunks;
     c->m32.erec = erec32;
     c->m32.E = E32;
     c->m32.ell_val = ell32;
+    c->m32.ell_col = col32;
+    c->m32.ell_val64 = val64p;
     c->mixed_ready = true;
   }
   return alloc_particles32(c);

@@ -110,6 +110,7 @@ struct Row32 {
   int c[ROW_REGS];
   float v[ROW_REGS];
   float3 g[ROW_REGS];  // (r, w, s) at column c
+  float p0;            // p of entry 0: the row's own word (its diagonal comes first)
 };
 
 __device__ __forceinline__ void stage_cols(const DevMesh& m, int row, Row32& R) {
@@ -129,6 +130,7 @@ __device__ __forceinline__ void stage_gathers(const DevMesh& m, const float* __r
     R.v[k] = k < R.len ? vp[k * 64] : 0.0f;
     const float4 t = q[R.c[k]];
     R.g[k] = make_float3(t.x, t.y, t.z);
+    if (k == 0) R.p0 = t.w;
   }
 }
 
@@ -309,7 +311,11 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   if (row) stage_cols(a.m, i, R);
   double pg, pd;
   partial_sums2(a.part_cur, a.part_cur + a.nchunks, a.nchunks, va, vb, pg, pd);
-  if (row) stage_gathers(a.m, a.val32, i, a.q_cur, R);
+  float di = 0;
+  if (row) {
+    stage_gathers(a.m, a.val32, i, a.q_cur, R);
+    di = a.d[i];
+  }
   if (done) return;
   // the sums are equal in every lane: scalar registers for them and what
   // follows from them leave the vector registers to the staged row
@@ -345,10 +351,9 @@ __global__ __launch_bounds__(BLOCK, 5) void k_ir_cg(IRArgs a) {
   const float bf = (float)beta, af = (float)alpha;
   double tg = 0, td = 0;
   if (row) {
-    // the row's own word (one of the gathered ones: the load hits the cache)
-    // and d, read late so that they hold no registers while the row is staged
-    const float4 own = a.q_cur[i];
-    const float di = a.d[i];
+    // the row's own word is entry 0 (the precision form's matrix keeps each
+    // row's diagonal first)
+    const float4 own = make_float4(R.g[0].x, R.g[0].y, R.g[0].z, R.p0);
     const float ri = own.x, wi = own.y;
     const float p = carry ? fmaf(bf, own.w, ri) : ri;
     const float s = carry ? fmaf(bf, own.z, wi) : wi;
@@ -411,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) void k_ir_update(IRArgs a, int pass_len) {
 // Arguments of inner iteration it (k_ir_start fills the parity-0 buffers: it = -1).
 IRArgs make_args(const DevMesh& m, const Mesh32& m32, IRBuffers& c, int it) {
   IRArgs a;
-  a.m = m;
+  a.m = diag_first(m, m32);
   a.val32 = m32.ell_val;
   a.x = c.x;
   a.b = c.b;

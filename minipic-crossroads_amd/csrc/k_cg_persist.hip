@@ -519,7 +519,7 @@ hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, 
       !c.part_b || !c.part_e || !c.st || !m32.ell_val || !epoch || !err || c.nchunks <= 0 || (int64_t)c.nchunks * BLOCK < m.nn)
     return hipErrorInvalidValue;
   PIRArgs a;
-  a.m = m;
+  a.m = diag_first(m, m32);
   a.val32 = m32.ell_val;
   a.x = c.x;
   a.b = c.b;

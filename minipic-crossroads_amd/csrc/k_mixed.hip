@@ -501,6 +501,36 @@ __global__ __launch_bounds__(BLOCK) void k_erec32(DevMesh m, float* __restrict__
   reinterpret_cast<int4*>(q)[3] = reinterpret_cast<const int4*>(src)[6];
 }
 
+// One row per thread: the row's diagonal entry (its first entry with column ==
+// row; padding entries, which repeat the row with value 0, come after it) moves
+// to position 0, the others keep their order.
+__global__ __launch_bounds__(BLOCK) void k_ell_diag_first(DevMesh m, int32_t* __restrict__ col,
+                                                          double* __restrict__ val64, float* __restrict__ val32,
+                                                          int* __restrict__ missing) {
+  const int row = blockIdx.x * BLOCK + threadIdx.x;
+  if (row >= m.nn) return;
+  const int slice = row >> 6, lane = row & 63;
+  const int len = m.ell_len[slice];
+  const int64_t base = m.ell_off[slice] + lane;
+  int kd = -1;
+  for (int k = 0; k < len && kd < 0; ++k)
+    if (m.ell_col[base + (int64_t)k * 64] == row) kd = k;
+  if (kd < 0) {
+    atomicOr(missing, 1);
+    kd = 0;
+  }
+  int o = 0;
+  for (int k = -1; k < len; ++k) {
+    const int src = k < 0 ? kd : k;
+    if (k == kd) continue;
+    const int64_t from = base + (int64_t)src * 64, to = base + (int64_t)o * 64;
+    col[to] = m.ell_col[from];
+    val64[to] = m.ell_val[from];
+    val32[to] = (float)m.ell_val[from];
+    ++o;
+  }
+}
+
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = (float)src[i];
@@ -615,6 +645,13 @@ hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_erec32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, out);
   return hipGetLastError();
 }
+hipError_t launch_ell_diag_first(const DevMesh& m, int32_t* col, double* val64, float* val32, int* missing,
+                                 hipStream_t s) {
+  if (m.nn <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ell_diag_first, dim3(grid_for(m.nn)), dim3(BLOCK), 0, s, m, col, val64, val32, missing);
+  return hipGetLastError();
+}
+
 hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   size_t g = (n + BLOCK - 1) / BLOCK;

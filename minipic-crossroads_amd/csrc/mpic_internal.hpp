@@ -115,8 +115,21 @@ struct DevParticles32 {
 struct Mesh32 {
   const float* erec;      // [ne][16] inverse Jacobian (9), node 0 (3), face neighbours (4 x int32), FP32
   float4* E;              // [ne]     field of the current potential (constant per tet)
-  const float* ell_val;   // FP32 stiffness values in the sliced ELL layout of DevMesh
+  // The precision form's copy of the sliced ELL matrix (same slices and
+  // lengths as DevMesh) with each row's diagonal entry first and the others in
+  // their stored order: columns, FP64 and FP32 values.
+  const int32_t* ell_col;
+  const double* ell_val64;
+  const float* ell_val;
 };
+// DevMesh whose ELL columns and FP64 values are the precision form's
+// diagonal-first copy.
+inline DevMesh diag_first(const DevMesh& m, const Mesh32& m32) {
+  DevMesh d = m;
+  d.ell_col = m32.ell_col;
+  d.ell_val = m32.ell_val64;
+  return d;
+}
 
 struct Species {
   double q[MPIC_MAX_SPECIES];
@@ -314,6 +327,10 @@ hipError_t launch_permute32(const DevParticles32& src, const DevParticles32& dst
 hipError_t launch_convert(const DevMesh& m, const DevParticles& P64, const DevParticles32& P32, int64_t off, int64_t n,
                           bool to32, hipStream_t s);
 hipError_t launch_to_float(const double* src, float* dst, size_t n, hipStream_t s);
+// The diagonal-first copy of m's ELL matrix; *missing (zeroed by the caller)
+// becomes nonzero when a row has no diagonal entry.
+hipError_t launch_ell_diag_first(const DevMesh& m, int32_t* col, double* val64, float* val32, int* missing,
+                                 hipStream_t s);
 hipError_t launch_erec32(const DevMesh& m, float* out, hipStream_t s);  // FP32 walk records
 // Ownership check: adds to *bad the live particles whose element does not
 // contain their position under the FP64 barycentric test.
