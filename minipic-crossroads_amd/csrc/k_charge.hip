@@ -14,16 +14,31 @@ __global__ __launch_bounds__(BLOCK) void k_node_charge(DevMesh m, unsigned long 
                                                        double* __restrict__ rhs) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= m.nn) return;
+  // the node's element slots (slot e*4+node: 4 digit words), NODE_BATCH at a
+  // time with all their loads in flight (indices clamped to the last slot,
+  // whose extra reads are not added); each slot is cleared for the next deposit
   fx128 s = m.bgfix ? m.bgfix[i] : fx128{0, 0};
   const int b = m.adj_ptr[i], e = m.adj_ptr[i + 1];
-  for (int k = b; k < e; ++k) {
-    unsigned long long* w = acc + (size_t)m.adj[k] * 4;  // slot e*4+node: 4 digit words
-    const ulonglong2 w01 = *reinterpret_cast<const ulonglong2*>(w);
-    const ulonglong2 w23 = *reinterpret_cast<const ulonglong2*>(w + 2);
-    const unsigned long long ws[4] = {w01.x, w01.y, w23.x, w23.y};
-    s = fx_add(s, dep_slot_value(ws));
-    *reinterpret_cast<ulonglong2*>(w) = ulonglong2{0, 0};
-    *reinterpret_cast<ulonglong2*>(w + 2) = ulonglong2{0, 0};
+  for (int k = b; k < e; k += NODE_BATCH) {
+    int a[NODE_BATCH];
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u) a[u] = m.adj[min(k + u, e - 1)];
+    ulonglong2 w01[NODE_BATCH], w23[NODE_BATCH];
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u) {
+      const unsigned long long* w = acc + (size_t)a[u] * 4;
+      w01[u] = *reinterpret_cast<const ulonglong2*>(w);
+      w23[u] = *reinterpret_cast<const ulonglong2*>(w + 2);
+    }
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u)
+      if (k + u < e) {
+        const unsigned long long ws[4] = {w01[u].x, w01[u].y, w23[u].x, w23[u].y};
+        s = fx_add(s, dep_slot_value(ws));
+        unsigned long long* w = acc + (size_t)a[u] * 4;
+        *reinterpret_cast<ulonglong2*>(w) = ulonglong2{0, 0};
+        *reinterpret_cast<ulonglong2*>(w + 2) = ulonglong2{0, 0};
+      }
   }
   rhs[i] = m.bnd[i] ? 0.0 : from_fixed(s, S);
 }

@@ -395,9 +395,21 @@ __global__ __launch_bounds__(BLOCK) void k_node_charge64(DevMesh m, const unsign
                                                          const double* __restrict__ bg, double* __restrict__ rhs) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= m.nn) return;
+  // the node's element slots, NODE_BATCH at a time with all their loads in
+  // flight (indices clamped to the last slot, whose extra reads are not added)
   unsigned long long s = 0;
   const int b = m.adj_ptr[i], e = m.adj_ptr[i + 1];
-  for (int k = b; k < e; ++k) s += acc[m.adj[k] >> 2];
+  for (int k = b; k < e; k += NODE_BATCH) {
+    int a[NODE_BATCH];
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u) a[u] = m.adj[min(k + u, e - 1)];
+    unsigned long long v[NODE_BATCH];
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u) v[u] = acc[a[u] >> 2];
+#pragma unroll
+    for (int u = 0; u < NODE_BATCH; ++u)
+      if (k + u < e) s += v[u];
+  }
   const double q = ldexp((double)(long long)s, -S);
   rhs[i] = m.bnd[i] ? 0.0 : (bg ? bg[i] : 0.0) + -0.25 * q;
 }

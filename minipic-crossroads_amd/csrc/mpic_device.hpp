@@ -25,6 +25,8 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
 // a clamped index, P > 0): a conditional load's register would have to be
 // waited for where the paths join.
 constexpr int PART_BATCH = 5;
+// Element slots of a node read together by the node-charge kernels.
+constexpr int NODE_BATCH = 8;
 __device__ __forceinline__ double thread_partials(const double* __restrict__ part, int P) {
   double v[PART_BATCH];
 #pragma unroll
