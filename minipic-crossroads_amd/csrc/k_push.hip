@@ -428,39 +428,62 @@ __global__ __launch_bounds__(BLOCK) void k_charge_bound(DevParticles P, int64_t 
 }
 
 // Final reduction of per-workgroup partials (one workgroup, integer sums and maxima).
-__global__ __launch_bounds__(BLOCK) void k_push_finish(const PushPartial* part, int nparts, PushPartial* out) {
-  fx128 ke{0, 0};
-  unsigned long long mx = 0;
-  long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
-  for (int i = threadIdx.x; i < nparts; i += BLOCK) {
-    const PushPartial q = part[i];
+// The sum of the workgroup partials (exact: fixed-point KE, integer counters,
+// and a maximum, so the order does not matter).  One workgroup of FINISH_T
+// threads; each thread's FINISH_U partials are loaded together, then waves
+// reduce by shuffles and one thread adds the waves' results.
+constexpr int FINISH_T = 1024;
+constexpr int FINISH_U = 4;
+
+struct FinishAcc {
+  fx128 ke;
+  unsigned long long mx;
+  long long c[6];
+  __device__ void add(const PushPartial& q) {
     ke = fx_add(ke, fx128{q.ke_lo, q.ke_hi});
     mx = q.ke_max_bits > mx ? q.ke_max_bits : mx;
-    c0 += q.crossings;
-    c1 += q.killed;
-    c2 += q.capped;
-    c3 += q.nonpos;
-    c4 += q.reflections;
-    c5 += q.relocated;
+    c[0] += q.crossings;
+    c[1] += q.killed;
+    c[2] += q.capped;
+    c[3] += q.nonpos;
+    c[4] += q.reflections;
+    c[5] += q.relocated;
   }
-  __shared__ PushPartial sh[BLOCK];
-  sh[threadIdx.x] = PushPartial{ke.lo, ke.hi, mx, c0, c1, c2, c3, c4, c5};
+  __device__ PushPartial out() const { return PushPartial{ke.lo, ke.hi, mx, c[0], c[1], c[2], c[3], c[4], c[5]}; }
+};
+
+__global__ __launch_bounds__(FINISH_T) void k_push_finish(const PushPartial* __restrict__ part, int nparts,
+                                                          PushPartial* out) {
+  FinishAcc acc{{0, 0}, 0, {0, 0, 0, 0, 0, 0}};
+  for (int i0 = threadIdx.x; i0 < nparts; i0 += FINISH_T * FINISH_U) {
+    PushPartial q[FINISH_U];
+#pragma unroll
+    for (int u = 0; u < FINISH_U; ++u) q[u] = part[min(i0 + u * FINISH_T, nparts - 1)];
+#pragma unroll
+    for (int u = 0; u < FINISH_U; ++u)
+      if (i0 + u * FINISH_T < nparts) acc.add(q[u]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    PushPartial o;
+    o.ke_lo = __shfl_xor(acc.ke.lo, off);
+    o.ke_hi = __shfl_xor(acc.ke.hi, off);
+    o.ke_max_bits = __shfl_xor(acc.mx, off);
+    o.crossings = __shfl_xor(acc.c[0], off);
+    o.killed = __shfl_xor(acc.c[1], off);
+    o.capped = __shfl_xor(acc.c[2], off);
+    o.nonpos = __shfl_xor(acc.c[3], off);
+    o.reflections = __shfl_xor(acc.c[4], off);
+    o.relocated = __shfl_xor(acc.c[5], off);
+    acc.add(o);
+  }
+  __shared__ PushPartial sh[FINISH_T / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc.out();
   __syncthreads();
   if (threadIdx.x == 0) {
-    PushPartial r = sh[0];
-    for (int w = 1; w < BLOCK; ++w) {
-      fx128 s = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
-      r.ke_lo = s.lo;
-      r.ke_hi = s.hi;
-      r.ke_max_bits = sh[w].ke_max_bits > r.ke_max_bits ? sh[w].ke_max_bits : r.ke_max_bits;
-      r.crossings += sh[w].crossings;
-      r.killed += sh[w].killed;
-      r.capped += sh[w].capped;
-      r.nonpos += sh[w].nonpos;
-      r.reflections += sh[w].reflections;
-      r.relocated += sh[w].relocated;
-    }
-    *out = r;
+    FinishAcc r{{0, 0}, 0, {0, 0, 0, 0, 0, 0}};
+    for (int w = 0; w < FINISH_T / 64; ++w) r.add(sh[w]);
+    *out = r.out();
   }
 }
 
@@ -506,7 +529,7 @@ hipError_t launch_move(const PushArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_push_finish(const PushPartial* part, int nparts, PushPartial* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_push_finish, dim3(1), dim3(BLOCK), 0, s, part, nparts, out);
+  hipLaunchKernelGGL(k_push_finish, dim3(1), dim3(FINISH_T), 0, s, part, nparts, out);
   return hipGetLastError();
 }
 hipError_t launch_deposit(const DevMesh& m, const DevParticles& P, int64_t n, const double* species, int S,
