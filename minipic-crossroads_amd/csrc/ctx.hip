@@ -588,7 +588,7 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       (rc = dalloc(c, &c->cg.st, 1)) || (rc = dalloc(c, &c->sb.count, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.start, (size_t)ne + 2)) || (rc = dalloc(c, &c->sb.cursor, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.blocksum, (size_t)ne / 2048 + 64)) || (rc = dalloc(c, &c->finish, 1)) ||
-      (rc = dalloc(c, &c->partial, MAX_PARTIALS)))
+      (rc = dalloc(c, &c->partial, PARTIAL_SLOTS)))
     return bail(rc);
   c->cg.nchunks = nchunks;
   if (hipHostMalloc((void**)&c->cg_h, sizeof(CGState)) != hipSuccess ||

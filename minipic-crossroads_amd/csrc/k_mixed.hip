@@ -582,11 +582,12 @@ hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStr
   hipLaunchKernelGGL(k_field32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, phi, E);
   return hipGetLastError();
 }
-// Default grid of k_push32: six rounds of the workgroups resident at once (five
-// per CU at 88 VGPRs: 7680 on 256 CUs), capped by MAX_PARTIALS and the number
-// of 256-particle chunks.  Measured at 100M particles: 4096 workgroups 6.26 ms,
-// 5120-8192 within 1% of each other at 5.89-5.98 ms (smaller ranges balance
-// the walk's uneven cost better).
+// Default grid of k_push32: ten rounds of the workgroups resident at once (six
+// per CU at 80 VGPRs: 15360 on 256 CUs), capped by PUSH32_MAX_GRID and the
+// number of 256-particle chunks.  Measured at 100M particles, time per step:
+// 6144 workgroups 9.19 ms, 7680 9.12, 8192 9.09, 9216 9.23, 12288 9.03, 15360
+// 9.02 (smaller ranges balance the walk's uneven cost better).
+constexpr int PUSH32_ROUNDS = 10;
 int push32_grid(int64_t n) {
   static int cached[16];
   int dev = 0;
@@ -597,7 +598,7 @@ int push32_grid(int64_t n) {
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push32, BLOCK, 0) == hipSuccess &&
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
           cus > 0)
-        cached[dev] = std::min(MAX_PARTIALS, 6 * per_cu * cus);
+        cached[dev] = std::min(PUSH32_MAX_GRID, PUSH32_ROUNDS * per_cu * cus);
       else
         cached[dev] = DEFAULT_GRID;
     }
