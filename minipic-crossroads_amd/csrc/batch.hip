@@ -151,7 +151,7 @@ PushArgs32 push_args(mpic_batch* b, double dt, unsigned long long* acc, int32_t*
 }
 
 int push_grid_used(mpic_batch* b) {
-  return b->push_grid > 0 ? std::min(b->push_grid, MAX_PARTIALS) : push32_grid(b->c->n);
+  return b->push_grid > 0 ? std::min(b->push_grid, PUSH32_MAX_GRID) : push32_grid(b->c->n);
 }
 
 // One step of every problem as a fixed sequence of launches (no host
@@ -445,7 +445,7 @@ int mpic_batch_tune(mpic_batch* b, double dt) {
   // push: workgroups (particles per workgroup), on the field of the current
   // potential, into scratch deposit and counts; the input set is not modified
   BCHK(b, launch_field32(c->m, c->phi, c->m32.E, s));
-  const int cand_g[4] = {1024, 2048, 4096, 8192};
+  const int cand_g[4] = {2048, 4096, 8192, 16384};
   best = 1e30f;
   for (int k = 0; k < 4; ++k) {
     BCHK(b, launch_zero(b->tune_acc, (size_t)c->m.ne * 8, s));

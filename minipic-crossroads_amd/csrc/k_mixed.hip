@@ -613,7 +613,8 @@ hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
   if (!a.m32.erec || !a.m32.E || !a.P.r || !a.Q.r || !a.P.elem || !a.Q.elem || !a.acc || !a.partial || !a.species ||
       !a.m.bucket)
     return hipErrorInvalidValue;
-  const unsigned g = a.grid > 0 ? (unsigned)(a.grid < MAX_PARTIALS ? a.grid : MAX_PARTIALS) : (unsigned)push32_grid(a.n);
+  const unsigned g =
+      a.grid > 0 ? (unsigned)(a.grid < PUSH32_MAX_GRID ? a.grid : PUSH32_MAX_GRID) : (unsigned)push32_grid(a.n);
   hipLaunchKernelGGL(k_push32, dim3(g), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }

@@ -307,7 +307,7 @@ struct PushArgs32 {
   unsigned long long* acc;  // [ne] element charge sums, 64-bit fixed point at 2^-S_charge
   PushPartial* partial;
   const double* species;
-  int grid;                 // workgroups (0: the default grid-stride size), at most MAX_PARTIALS
+  int grid;                 // workgroups (0: the default grid-stride size), at most PUSH32_MAX_GRID
 };
 hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStream_t s);
 hipError_t launch_push32(const PushArgs32& a, hipStream_t s);

@@ -169,7 +169,7 @@ def test_batch_tune(mp):
     for k in range(len(SIDES)):
         assert np.array_equal(b.phi(k), phi0[k])  # tuning advances nothing
     st = b.step(0.05)
-    assert st["tuned"] == 1 and st["push_grid"] in (1024, 2048, 4096, 8192)
+    assert st["tuned"] == 1 and st["push_grid"] in (2048, 4096, 8192, 16384)
     assert st["solve_threads"] in (256, 512, 1024) and st["all_converged"] == 1
     assert b.check_ownership() == 0
     b.close()
