@@ -6,7 +6,11 @@
                  on the reference's 2x2x2 brick of 5-tet hexes on [-1,1]^3
                  (mesh.cpp:83-224), restated in pure Python from the reference
                  text: seed formula, 10 burned draws, rejection, refToPhysical,
-                 set_vel with math.sqrt/math.exp.
+                 set_vel with math.sqrt/math.exp; "two_stream" holds the
+                 reference's two-stream fills (main.cpp:98-106: beams of
+                 type 0 at vel (+-0.5, 0, 0), temp -0.05, then type 2 at rest),
+                 whose velocities come from set_vel's temp < 0 branch
+                 (particle_fill.hpp:106-111) with math.sin.
   c1_solve.npz   the C1 problem's electron charge vector (oracle deposit) and the
                  Dirichlet Poisson solution from scipy.sparse.linalg.spsolve.
 
@@ -99,6 +103,9 @@ def fill_particle(els, elem, ipart, ptype, nppe, density, vel, temp, mass):
             ph[d] += wts[i] * X[i][d]
     if temp == 0:
         v = list(vel)
+    elif temp < 0:
+        # beam with a velocity perturbation; no random draws
+        v = [vel[0] * (1. + temp * math.sin(1. * math.pi * ph[0])), 0.0, 0.0]
     else:
         while True:
             r = dr()
@@ -124,6 +131,17 @@ def fill_kat():
         for elem in (0, 1, 17, 39):
             for ip in range(3):
                 out["particles"].append(fill_particle(els, elem, ip, ptype, 3, density, (0, 0, 0), 1.0, 1.0))
+    # two-stream fills on the same brick: two beams of 2 per tet at half density,
+    # then the ions (type 2, 4 per tet, at rest)
+    out["two_stream"] = []
+    for (nppe, ptype, dens, vel, temp) in ((2, 0, density / 2, (0.5, 0.0, 0.0), -0.05),
+                                           (2, 0, density / 2, (-0.5, 0.0, 0.0), -0.05),
+                                           (4, 2, density, (0.0, 0.0, 0.0), 0.0)):
+        fill = {"nppe": nppe, "type": ptype, "density": dens.hex(), "vel": list(vel), "temp": temp, "particles": []}
+        for elem in (0, 3, 17, 39):
+            for ip in range(nppe):
+                fill["particles"].append(fill_particle(els, elem, ip, ptype, nppe, dens, vel, temp, 1.0))
+        out["two_stream"].append(fill)
     return out
 
 

@@ -4,9 +4,17 @@ independent computations (the reference ships no tests or golden vectors).
 Each oracle function and the check that pins it:
   LCG / drand                 -> big-int known answers (golden/lcg_kat.json)
   fill                        -> independent pure-Python restatement
-                                 (golden/fill_kat.json), plus moments
+                                 (golden/fill_kat.json), plus moments; the
+                                 two-stream velocity branch (temp < 0) -> the
+                                 same restatement's "two_stream" fills, plus
+                                 v = (vel (1 + temp sin(pi x)), 0, 0) on every
+                                 particle with math.sin
   mesh generators, geometry   -> volumes, det, J*Jinv = I, face convention,
                                  neighbour symmetry, boundary counts (mesh.cpp:641-647)
+  node mappings 1-3           -> closed forms: 1 = rotation about z by
+                                 (1-|x|)(1-|y|) (radius and z kept), 2 = cube
+                                 surface onto the unit sphere (signs kept),
+                                 3 = x -> sign(x) x^2
   stiffness / CSR             -> hand element matrix, symmetry, row sums,
                                  linear-field null space, scipy solve
   CG + dot tree               -> scipy spsolve (golden/c1_solve.npz), numpy dot
@@ -65,6 +73,41 @@ def test_fill_known_answers(oracle):
             assert p.xr[d, idx] == float.fromhex(q["xr"][d])
             assert p.v[d, idx] == float.fromhex(q["v"][d])
         assert p.w[idx] == float.fromhex(q["w"])
+
+
+def test_fill_two_stream_known_answers(oracle):
+    """set_vel's temp < 0 branch (particle_fill.hpp:106-111) through the
+    reference's two-stream fills (main.cpp:98-106)."""
+    kat = json.load(open(os.path.join(GOLD, "fill_kat.json")))["two_stream"]
+    m = oracle.Mesh(2, 2, 2, 5, lo=(-1, -1, -1), hi=(1, 1, 1))
+    p = oracle.Particles(m.ne * 8)
+    offsets = []
+    for f in kat:
+        offsets.append(p.n)
+        p.fill(m, f["nppe"], f["type"], float.fromhex(f["density"]), vel=f["vel"], temp=f["temp"])
+    assert p.n == m.ne * 8
+    for off, f in zip(offsets, kat):
+        for q in f["particles"]:
+            idx = off + q["elem"] * f["nppe"] + q["ipart"]
+            assert p.elem[idx] == q["elem"] and p.type[idx] == q["type"]
+            for d in range(3):
+                assert p.x[d, idx] == float.fromhex(q["x"][d])
+                assert p.xr[d, idx] == float.fromhex(q["xr"][d])
+                assert p.v[d, idx] == float.fromhex(q["v"][d])
+            assert p.w[idx] == float.fromhex(q["w"])
+    # every beam particle: v = (vel0 (1 + temp sin(pi x0)), 0, 0); the beams
+    # share positions (same seeds) and have opposite velocities
+    nb = m.ne * 2
+    for k, f in enumerate(kat[:2]):
+        sl = slice(offsets[k], offsets[k] + nb)
+        assert not p.v[1:, sl].any()
+        want = [f["vel"][0] * (1.0 + f["temp"] * math.sin(math.pi * x)) for x in p.x[0, sl]]
+        assert np.array_equal(p.v[0, sl], np.array(want))
+        assert abs(p.v[0, sl]).min() > 0.5 * (1 - 0.05) - 1e-15 and abs(p.v[0, sl]).max() < 0.5 * (1 + 0.05) + 1e-15
+    assert np.array_equal(p.x[:, :nb], p.x[:, nb:2 * nb])
+    assert np.array_equal(p.v[0, :nb], -p.v[0, nb:2 * nb])
+    # ions at rest
+    assert not p.v[:, 2 * nb:p.n].any()
 
 
 def test_fill_statistics(oracle):
@@ -135,6 +178,45 @@ def test_mesh_geometry(oracle, split, box):
     assert walls == 2 * (2 * nx * ny + 2 * nx * nz + 2 * ny * nz)
     nb_nodes = m.nn - (nx - 1) * (ny - 1) * (nz - 1)
     assert m.nbnd == nb_nodes
+
+
+@pytest.mark.parametrize("split", [5, 6])
+def test_node_mappings_closed_form(oracle, split):
+    """Mesh::apply_mapping (mesh.cpp:763-792) on the reference's [-1,1]^3 box,
+    checked node by node against closed-form properties of each map."""
+    box = dict(lo=(-1, -1, -1), hi=(1, 1, 1))
+    n = 4
+    X = oracle.Mesh(n, n, n, split, **box).nodes
+    x, y, z = X.T
+    # 1: rotation about the z axis by r = (1-|x|)(1-|y|)
+    Y = oracle.Mesh(n, n, n, split, mapping=1, **box).nodes
+    assert np.allclose(np.hypot(Y[:, 0], Y[:, 1]), np.hypot(x, y), rtol=0, atol=1e-15)
+    assert np.array_equal(Y[:, 2], z)
+    r = (1 - abs(x)) * (1 - abs(y))
+    moved = np.hypot(x, y) > 0
+    turn = np.angle((Y[:, 0] + 1j * Y[:, 1]) / (x + 1j * y + ~moved))
+    assert np.allclose(turn[moved], r[moved], rtol=0, atol=1e-14)
+    assert r.max() > 0.5  # the interior really turns
+    on_xy_wall = (abs(x) == 1) | (abs(y) == 1)
+    assert np.array_equal(Y[on_xy_wall], X[on_xy_wall])
+    # 2: the cube's surface onto the unit sphere, signs kept, centre fixed
+    Y = oracle.Mesh(n, n, n, split, mapping=2, **box).nodes
+    surf = (abs(X) == 1).any(axis=1)
+    assert surf.sum() == (n + 1) ** 3 - (n - 1) ** 3
+    assert np.allclose(np.linalg.norm(Y[surf], axis=1), 1.0, rtol=0, atol=1e-15)
+    assert np.array_equal(np.sign(Y), np.sign(X))
+    assert (np.linalg.norm(Y[~surf], axis=1) < 1).all()
+    # on a coordinate axis the map is the identity (the other two terms vanish)
+    axis = (X == 0).sum(axis=1) >= 2
+    assert np.array_equal(Y[axis], X[axis])
+    # 3: x -> sign(x) x^2, y and z kept
+    Y = oracle.Mesh(n, n, n, split, mapping=3, **box).nodes
+    assert np.array_equal(Y[:, 0], np.where(x < 0, -(x * x), x * x))
+    assert np.array_equal(Y[:, 1:], X[:, 1:])
+    # the topology does not depend on the mapping
+    for k in (1, 2, 3):
+        assert np.array_equal(oracle.Mesh(n, n, n, split, mapping=k, **box).elem_nodes,
+                              oracle.Mesh(n, n, n, split, **box).elem_nodes)
 
 
 def test_element_stiffness_by_hand(oracle):
