@@ -115,80 +115,99 @@ struct RareCount {
   int capped, nonpos, reflections, relocated, killed;
 };
 
-// The tet walk of MoveParticle (particle_move.hpp:103-124,136-225) in FP32.
+// The tet walk of MoveParticle (particle_move.hpp:103-124,136-225) in FP32,
+// over the remaining time tau = dt * time_remaining instead of the fraction.
+// The exit time through face f is a_f / b_f with b_f > 0:
+//   zeta (face 3): R2 / -VR2;  eta (face 0): R1 / -VR1;  xi (face 2): R0 / -VR0;
+//   sum (face 1): (1 - R0 - R1 - R2) / (VR0 + VR1 + VR2).
+// Starting from the candidate tau / 1 (no face), a face replaces the current
+// candidate A / B when b_f > 0 and a_f B < A b_f, tested in the reference's
+// order (zeta, eta, xi, sum).  Against tau this is the reference's "the end
+// point lies beyond the face" test, between faces its strictly-smaller
+// replacement; the products avoid a division per face, and the chosen time is
+// A * rcp(B) (1 ulp).  The fraction s of the reference is t / tau.  The
+// neighbour across the chosen face is selected with the face (a select per
+// face keeps the element record in registers).  At WALK_CAP face hits the walk
+// stops (counted as capped).
 // g holds the record of element e on entry and of the final element on return.
 __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict__ erec, int& e, ElemRec32& g,
                                        float X[3], float R[3], float V[3], float VR[3], float dt, int kill,
                                        int& crossings, RareCount& rare) {
-  float tr = 1.0f;
-  int iters = 0;
-  while (tr > 0.0f) {
-    if (iters++ >= WALK_CAP) {
+  float tau = dt;
+  for (int hits = 0;; ++hits) {
+    if (hits >= WALK_CAP) {
       atomicAdd(&rare.capped, 1);
       break;
     }
-    const float ldt = dt * tr;
-    float s = 1.0f;
-    int face = -1;
-    float dxr[3], xf[3];
+    float A = tau, B = 1.0f;
+    int face = -1, nbs = -1;  // the face taken and the neighbour across it
+    {
+      const float a = R[2], b = -VR[2];
+      const bool take = b > 0.0f && a * B < A * b;
+      A = take ? a : A;
+      B = take ? b : B;
+      face = take ? 3 : face;
+      nbs = take ? g.nb[3] : nbs;
+    }
+    {
+      const float a = R[1], b = -VR[1];
+      const bool take = b > 0.0f && a * B < A * b;
+      A = take ? a : A;
+      B = take ? b : B;
+      face = take ? 0 : face;
+      nbs = take ? g.nb[0] : nbs;
+    }
+    {
+      const float a = R[0], b = -VR[0];
+      const bool take = b > 0.0f && a * B < A * b;
+      A = take ? a : A;
+      B = take ? b : B;
+      face = take ? 2 : face;
+      nbs = take ? g.nb[2] : nbs;
+    }
+    {
+      const float a = 1.0f - R[0] - R[1] - R[2], b = VR[0] + VR[1] + VR[2];
+      const bool take = b > 0.0f && a * B < A * b;
+      A = take ? a : A;
+      B = take ? b : B;
+      face = take ? 1 : face;
+      nbs = take ? g.nb[1] : nbs;
+    }
+    if (face < 0) {
+      // no face before the end of the step
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        X[d] += tau * V[d];
+        R[d] += tau * VR[d];
+      }
+      break;
+    }
+    const float t = A * __builtin_amdgcn_rcpf(B);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      dxr[d] = ldt * VR[d];
-      xf[d] = R[d] + dxr[d];
+      X[d] += t * V[d];
+      R[d] += t * VR[d];
     }
-    if (VR[2] < 0 && xf[2] < 0) {
-      face = 3;
-      s = -R[2] / dxr[2];
+    if (nbs >= 0) {
+      e = nbs;
+      crossings++;
+      g = load_erec32(erec, e);
+      const float dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
+      p2r_vecf(g.J, dl, R);
+    } else {
+      if (kill) return false;
+      atomicAdd(&rare.reflections, 1);
+      const double* nrm = m.normals + (size_t)e * 12 + face * 3;
+      const float n0 = (float)nrm[0], n1 = (float)nrm[1], n2 = (float)nrm[2];
+      const float vn = V[0] * n0 + V[1] * n1 + V[2] * n2;
+      V[0] -= 2 * vn * n0;
+      V[1] -= 2 * vn * n1;
+      V[2] -= 2 * vn * n2;
     }
-    if (VR[1] < 0 && xf[1] < 0) {
-      const float sfy = -R[1] / dxr[1];
-      if (sfy < s) {
-        face = 0;
-        s = sfy;
-      }
-    }
-    if (VR[0] < 0 && xf[0] < 0) {
-      const float sfx = -R[0] / dxr[0];
-      if (sfx < s) {
-        face = 2;
-        s = sfx;
-      }
-    }
-    if (VR[0] + VR[1] + VR[2] > 0 && xf[0] + xf[1] + xf[2] > 1) {
-      const float sfs = (1 - R[0] - R[1] - R[2]) / (dxr[0] + dxr[1] + dxr[2]);
-      if (sfs < s) {
-        face = 1;
-        s = sfs;
-      }
-    }
-    const float sl = s * ldt;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      X[d] += sl * V[d];
-      R[d] += sl * VR[d];
-    }
-    if (s < 1) {
-      const int e2 = face == 0 ? g.nb[0] : face == 1 ? g.nb[1] : face == 2 ? g.nb[2] : g.nb[3];
-      if (e2 >= 0) {
-        e = e2;
-        crossings++;
-        g = load_erec32(erec, e);
-        const float dl[3] = {X[0] - g.n0[0], X[1] - g.n0[1], X[2] - g.n0[2]};
-        p2r_vecf(g.J, dl, R);
-      } else {
-        if (kill) return false;
-        atomicAdd(&rare.reflections, 1);
-        const double* nrm = m.normals + (size_t)e * 12 + face * 3;
-        const float n0 = (float)nrm[0], n1 = (float)nrm[1], n2 = (float)nrm[2];
-        const float vn = V[0] * n0 + V[1] * n1 + V[2] * n2;
-        V[0] -= 2 * vn * n0;
-        V[1] -= 2 * vn * n1;
-        V[2] -= 2 * vn * n2;
-      }
-      p2r_vecf(g.J, V, VR);
-    }
-    if (!(s > 0)) atomicAdd(&rare.nonpos, 1);
-    tr -= s * tr;
+    p2r_vecf(g.J, V, VR);
+    if (!(t > 0.0f)) atomicAdd(&rare.nonpos, 1);
+    tau -= t;
+    if (!(tau > 0.0f)) break;
   }
   return true;
 }
