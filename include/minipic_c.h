@@ -241,6 +241,11 @@ typedef struct {
   int all_converged;
   int64_t num_particles;
   double ms_step;       /* device time of the last step */
+  int64_t crossings;    /* element-face crossings of the last step, all problems */
+  int64_t capped;       /* particles stopped by the crossing cap (the step then returns MPIC_ERR_WALK_CAP) */
+  int64_t nonpositive;  /* sub-steps with a non-positive exit time */
+  int ke_rescaled;      /* 1: the largest kinetic-energy term outgrew the fixed-point scale set at fill;
+                           the per-problem sums were redone at a safe scale and the graphs recaptured */
 } mpic_batch_stats;
 
 /* Builds the ensemble from host meshes (mpic_mesh_brick), borrowed for the call. */
@@ -250,7 +255,9 @@ void mpic_batch_destroy(mpic_batch* b);
 int mpic_batch_fill(mpic_batch* b, const mpic_batch_fill_desc* fills);
 /* Times the candidate tile shapes on the current state (nothing is advanced) and keeps the fastest. */
 int mpic_batch_tune(mpic_batch* b, double dt);
-/* One PIC::time_step of every problem.  With graphs enabled the step is a replay. */
+/* One PIC::time_step of every problem.  With graphs enabled the step is a replay.
+ * Returns MPIC_ERR_WALK_CAP, after completing the step, when a particle hit the
+ * crossing cap (as mpic_time_step does). */
 int mpic_batch_step(mpic_batch* b, double dt, mpic_batch_stats* st);
 /* enable = 1: capture the step (dt of the next call) and replay it from then on. */
 int mpic_batch_graph(mpic_batch* b, int enable);

@@ -510,10 +510,26 @@ int mpic_batch_step(mpic_batch* b, double dt, mpic_batch_stats* st) {
   }
   BCHK(b, hipEventRecord(b->ev1, s));
   after_step(b);
+  BCHK(b, hipMemcpyAsync(c->finish_h, c->finish, sizeof(PushPartial), hipMemcpyDeviceToHost, s));
   BCHK(b, hipStreamSynchronize(s));
   float ms = 0;
   BCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->ms_step = ms;
+  // The push's totals (crossing counters, largest KE term).  The per-problem KE
+  // sums of this step used the scale fixed at fill; if the largest term has
+  // outgrown it they are summed again at a safe scale, which also becomes the
+  // scale of later steps (the captured graphs hold the old one: recaptured).
+  const PushPartial fin = *c->finish_h;
+  double kmax;
+  std::memcpy(&kmax, &fin.ke_max_bits, 8);
+  const bool rescale = scale_for_bound((double)c->n * kmax) < b->S_ke;
+  if (rescale) {
+    b->S_ke = scale_for_bound((double)c->n * kmax * 256.0);
+    drop_graphs(b);
+    BCHK(b, launch_batch_stats(c->P32[c->cur32].r, b->d_part_off, b->d_chunk_off, b->nchunks, c->species, b->S_ke,
+                               b->d_stat_part, b->d_ke, b->d_count, b->nprob, s));
+    BCHK(b, hipStreamSynchronize(s));
+  }
   if (st) {
     std::vector<int> it(b->nprob), cv(b->nprob);
     BCHK(b, hipMemcpy(it.data(), b->d_iters, sizeof(int) * b->nprob, hipMemcpyDeviceToHost));
@@ -526,7 +542,12 @@ int mpic_batch_step(mpic_batch* b, double dt, mpic_batch_stats* st) {
     st->all_converged = std::all_of(cv.begin(), cv.end(), [](int v) { return v != 0; }) ? 1 : 0;
     st->num_particles = c->n;
     st->ms_step = ms;
+    st->crossings = fin.crossings;
+    st->capped = fin.capped;
+    st->nonpositive = fin.nonpos;
+    st->ke_rescaled = rescale ? 1 : 0;
   }
+  if (fin.capped > 0) return bfail(b, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
   return MPIC_OK;
 }
 

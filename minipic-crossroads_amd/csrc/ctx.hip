@@ -710,7 +710,6 @@ int mpic_particles_reserve(mpic_ctx* c, int64_t cap) {
 }
 
 int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
-  if (c) c->perm_pending = false;  // the list is replaced
   if (!c || !h || n < 0 || !h->x || !h->x_ref || !h->v || !h->elem || !h->weight || !h->type || h->stride < n)
     return fail(c, MPIC_ERR_ARG, "mpic_particles_upload: bad arguments");
   if (n > c->cap) return fail(c, MPIC_ERR_CAPACITY, "mpic_particles_upload: capacity exceeded");
@@ -732,6 +731,8 @@ int mpic_particles_upload(mpic_ctx* c, const mpic_soa_host* h, int64_t n) {
     r.type = (uint32_t)h->type[p];
     r.pad = 0;
   }
+  // every check passed: the list is replaced (a pending ranking of the old list is dropped)
+  c->perm_pending = false;
   DevParticles& P = c->P[c->cur];
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (n) {

@@ -374,8 +374,10 @@ int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   st.ms_sort = c->ms[T_SORT];
   st.ms_ke = c->ms[T_KE];
   if (out) *out = st;
-  // A particle stopped by the crossing cap is still placed by the FP64 location
-  // test; it is reported in the move statistics, not as an error.
+  // The same contract as the FP64 form: the step completes (a particle stopped
+  // by the crossing cap stays where the cap found it, inside its element after
+  // the FP64 location test), the statistics count it, and the call reports it.
+  if (r.capped > 0) return fail(c, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
   return MPIC_OK;
 }
 

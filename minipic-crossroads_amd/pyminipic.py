@@ -63,7 +63,8 @@ class BatchFill(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("push_grid", C.c_int), ("solve_threads", C.c_int), ("tuned", C.c_int), ("graph", C.c_int),
                 ("max_inner_iters", C.c_int), ("all_converged", C.c_int), ("num_particles", C.c_int64),
-                ("ms_step", C.c_double)]
+                ("ms_step", C.c_double), ("crossings", C.c_int64), ("capped", C.c_int64),
+                ("nonpositive", C.c_int64), ("ke_rescaled", C.c_int)]
 
 
 class MoveStats(C.Structure):
@@ -306,10 +307,16 @@ class Context:
     def accel(self, dt, scale=1.0):
         self._chk(lib().mpic_accel(self.h, dt, scale))
 
-    def move(self, dt):
+    def move(self, dt, check=True):
+        """MoveParticle::execute.  check=False returns (status, stats) instead of
+        raising on a non-OK status (e.g. ERR_WALK_CAP, which still completes the move)."""
         st = MoveStats()
-        self._chk(lib().mpic_move(self.h, dt, C.byref(st)))
-        return {k: getattr(st, k) for k, _ in MoveStats._fields_}
+        rc = lib().mpic_move(self.h, dt, C.byref(st))
+        out = {k: getattr(st, k) for k, _ in MoveStats._fields_}
+        if not check:
+            return rc, out
+        self._chk(rc)
+        return out
 
     def compute_ke(self):
         ke = C.c_double()
@@ -321,9 +328,14 @@ class Context:
         self._chk(lib().mpic_sort_by_element(self.h, _p(start)))
         return start
 
-    def time_step(self, dt):
+    def time_step(self, dt, check=True):
+        """PIC::time_step.  check=False returns (status, stats) instead of raising
+        on a non-OK status (e.g. ERR_WALK_CAP, which still completes the step)."""
         st = StepStats()
-        self._chk(lib().mpic_time_step(self.h, dt, C.byref(st)))
+        rc = lib().mpic_time_step(self.h, dt, C.byref(st))
+        if not check:
+            return rc, st.as_dict()
+        self._chk(rc)
         return st.as_dict()
 
     def rhs(self):
@@ -422,10 +434,15 @@ class Batch:
     def graph(self, enable):
         self._chk(lib().mpic_batch_graph(self.h, int(enable)))
 
-    def step(self, dt):
+    def step(self, dt, check=True):
+        """One time step of every problem; check=False returns (status, stats)."""
         st = BatchStats()
-        self._chk(lib().mpic_batch_step(self.h, dt, C.byref(st)))
-        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+        rc = lib().mpic_batch_step(self.h, dt, C.byref(st))
+        out = {k: getattr(st, k) for k, _ in BatchStats._fields_}
+        if not check:
+            return rc, out
+        self._chk(rc)
+        return out
 
     def problem_stats(self):
         n = self.nprob

@@ -26,6 +26,11 @@ Each oracle function and the check that pins it:
   walk                        -> free streaming with specular walls (unfolded
                                  closed form), containment, conservation
   KE                          -> direct numpy sum
+  failure paths               -> over-capacity fill refused with n unchanged
+                                 (particle_fill.hpp:61-64), CG max_iter exit
+                                 and zero-residual return (CG_solve.hpp:71-113),
+                                 the walk's crossing cap with every capped
+                                 particle still inside its element
   periodic mesh (two-stream)  -> joined faces are the same (y, z) triangle on
                                  opposite x walls, image rows, analytic
                                  periodic Poisson solution (second order),
@@ -546,3 +551,46 @@ def test_periodic_bicgstab_against_scipy(oracle):
     x = np.zeros(m.nn)
     s, it, rr = oracle.bicgstab(m, b, x, tol=1e-12)
     assert s == 0 and np.abs(x - xs).max() <= 1e-9 * np.abs(xs).max()
+
+
+def test_failure_paths(oracle):
+    """Over-capacity fill, CG exits and the walk cap on the oracle itself."""
+    m = oracle.Mesh(4, 4, 4, 6)
+    # ParticleFill refuses a fill beyond the capacity and leaves the list alone
+    p = oracle.Particles(m.ne * 4 + 7).fill(m, 4, 0, DENSITY, temp=0.02, seed_offset=1)
+    with pytest.raises(ValueError):
+        p.fill(m, 4, 0, DENSITY, temp=0.02)
+    assert p.n == m.ne * 4
+    # CG: one iteration then the max_iter exit; a zero residual returns at once
+    rhs = oracle.deposit(m, p)
+    phi = np.zeros(m.nn)
+    ok, it, rr = oracle.cg(m, rhs, phi, 1e-10, 1)
+    assert not ok and it == 1 and rr > 1e-3
+    # one CG step from 0 is x = (r.r / r.A r) r, r = b
+    r = rhs
+    Ar = m.spmv(r)
+    assert np.allclose(phi, (r @ r) / (r @ Ar) * r, rtol=1e-13, atol=0)
+    ok, it, rr = oracle.cg(m, np.zeros(m.nn), np.zeros(m.nn), 1e-10, 1000)
+    assert ok and it == 0 and rr == 0.0
+    # the walk cap: a few particles sent 500 box lengths stop after WALK_CAP
+    # face hits, still inside the element they are in
+    m2 = oracle.Mesh(2, 2, 2, 6)
+    q = oracle.Particles(4)
+    X = m2.nodes[m2.elem_nodes]
+    starts = np.array([[0.3, 0.2, 0.1], [0.6, 0.7, 0.2], [0.1, 0.4, 0.8]])
+    for i, x in enumerate(starts):
+        for e in range(m2.ne):
+            xr = m2.jinv[e] @ (x - X[e, 0])
+            if min(1 - xr.sum(), *xr) >= 0:
+                q.x[:, i], q.xr[:, i], q.elem[i] = x, xr, e
+                break
+    q.v[:, :3] = np.array([[3.7, -2.9, 1.3], [1.0, 2.0, 3.0], [-5.0, 0.5, 2.0]]).T
+    q.w[:3] = 1.0
+    q.n = 3
+    st = oracle.push(m2, q, np.zeros(m2.nn), 100.0)
+    assert st["capped"] == 3 and st["killed"] == 0
+    for i in range(3):
+        e = q.elem[i]
+        xr = m2.jinv[e] @ (q.x[:, i] - X[e, 0])
+        assert min(1 - xr.sum(), *xr) > -1e-12
+        assert np.allclose(xr, q.xr[:, i], atol=1e-12)
