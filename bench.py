@@ -38,6 +38,31 @@ DENSITY = 2.0 / 3.0 * math.pi * math.pi  # main.cpp:68
 # weight, species; writes x, x_ref, v, elem.
 PUSH_BYTES_FP64 = 3 * 24 + 4 + 8 + 1 + 3 * 24 + 4        # 161 B
 PUSH_BYTES_MIXED = 3 * 12 + 4 + 8 + 1 + 3 * 12 + 4        # 89 B (FP32 state, FP64 weight)
+# Infinity Cache (MALL) of the MI355X: a matrix below this size can stay resident
+MALL_BYTES = 256 << 20
+
+
+def cg_bytes_per_iter(n, nnz, word):
+    """Algorithmic bytes of one CG iteration: the CSR SpMV (values of `word`
+    bytes, int32 columns and row offsets, the input vector read once and the
+    output written once) plus five vector passes (p update, p.q, x update, r
+    update, r.r) of `word` bytes per row.  FP64 CG: word 8 (the SpMV part is
+    12 nnz + 4 (n+1) + 16 n); FP32 inner CG: word 4."""
+    spmv = (word + 4) * nnz + 4 * (n + 1) + 2 * word * n
+    return spmv, spmv + 5 * word * n
+
+
+def cg_roofline(sm, n, nnz, word, kernel):
+    spmv, per_iter = cg_bytes_per_iter(n, nnz, word)
+    it, ms = sm["cg_iters_mean"], sm["cg_ms_per_step"]
+    achieved = per_iter * it / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"kernel": kernel, "bound": "hbm", "bytes_per_iter": per_iter, "spmv_bytes_per_iter": spmv,
+            "iters_per_step": it, "ms_per_step": ms, "us_per_iter": ms * 1e3 / it if it else None,
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "matrix_bytes": (word + 4) * nnz,
+            "matrix_fits_infinity_cache": (word + 4) * nnz < MALL_BYTES,
+            "time_covers": "the whole solve of a step (FP64 form: CG; precision form: FP32 inner CG and the "
+                           "FP64 residuals and updates of the refinement), host waits included"}
 
 
 def measured_traffic(kernel):
@@ -88,7 +113,6 @@ def run_gpu(args, form):
     npart = ctx.num_particles()
     ctx.weight_charge()
     ctx.solve(1e-10, 1000)
-    phi_fill = ctx.phi()
     log("[%s] filled %d particles" % (name, npart))
     for i in range(args.warmup):
         st = ctx.time_step(args.dt)
@@ -109,7 +133,7 @@ def run_gpu(args, form):
             name, i, st["cg_iters"], st["ms_solve"], st["ms_deposit"], st["ms_push"], st["ms_sort"],
             st["kinetic_energy"]))
     out = dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nnz,
-               phi_fill=phi_fill, phi_end=ctx.phi(), rhs_end=ctx.rhs(), owner_mismatch=ctx.check_ownership())
+               owner_mismatch=ctx.check_ownership())
     ctx.close()
     return out
 
@@ -122,7 +146,64 @@ def summary(g, K, bytes_per_particle):
                 push_gbs=achieved, cg_iters_mean=sum(s["cg_iters"] for s in st) / K,
                 cg_ms_per_step=sum(s["ms_solve"] for s in st) / K,
                 sort_ms_per_step=sum(s["ms_sort"] for s in st) / K,
+                sort_ms_per_reorder=(sum(s["ms_sort"] for s in st) / max(1, sum(1 for s in st if s["ms_sort"] > 0))),
+                field_ms_per_step=sum(s["ms_field"] for s in st) / K,
                 deposit_ms_per_step=sum(s["ms_deposit"] for s in st) / K)
+
+
+def run_error_curves(args):
+    """The precision form's error against the FP64 form step by step, with a
+    control: the FP64 form started from the same fill rounded to FP32 and back
+    (position and velocity; the reference position recomputed in FP64 from the
+    rounded position).  When the control drifts from the FP64 run as fast as
+    the precision form does, the drift is the divergence of the trajectories
+    (particles put on the other side of a face by a rounding), not a bias of
+    the precision form.  Untimed; --err-steps steps from the fill solve."""
+    import numpy as np
+    mp = load("pyminipic", os.path.join(PKG, "pyminipic.py"))
+    n = args.nx
+    mesh = mp.Mesh(n, n, n, 6, background=DENSITY)
+    runs = {}
+    for name, form, roundtrip in (("fp64", mp.FORM_FP64, False), ("mixed", mp.FORM_MIXED, False),
+                                  ("fp64_fp32_roundtrip", mp.FORM_FP64, True)):
+        ctx = mp.Context(mesh.desc, mp.default_config(sort_period=args.sort_period, cg_tol=1e-10, form=form))
+        ctx.reserve(mesh.ne * args.nppe)
+        ctx.fill(args.nppe, 0, DENSITY, temp=2 * 0.1 ** 2, seed_offset=1, pert_amp=0.05)
+        if roundtrip:
+            ctx.set_form(mp.FORM_MIXED)
+            ctx.set_form(mp.FORM_FP64)
+        ctx.weight_charge()
+        ctx.solve(1e-10, 1000)
+        phis, ke = [ctx.phi()], []
+        for _ in range(args.err_steps):
+            st = ctx.time_step(args.dt)
+            phis.append(ctx.phi())
+            ke.append(st["kinetic_energy"])
+        runs[name] = dict(phi=phis, ke=ke, rhs=ctx.rhs(), owner=ctx.check_ownership())
+        ctx.close()
+        log("[errors] %s: %d steps" % (name, args.err_steps))
+    ref = runs["fp64"]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    curve = lambda r: [float("%.3g" % rel(a, b)) for a, b in zip(r["phi"], ref["phi"])]
+    fe = lambda r: float(np.dot(r["phi"][-1], r["rhs"]))
+    m, c = runs["mixed"], runs["fp64_fp32_roundtrip"]
+    return {
+        "phi_l2_rel_same_particles": rel(m["phi"][0], ref["phi"][0]),
+        "phi_l2_rel_after_steps": rel(m["phi"][-1], ref["phi"][-1]),
+        "field_energy_rel_after_steps": abs(fe(m) / fe(ref) - 1.0),
+        "kinetic_energy_rel_after_steps": abs(m["ke"][-1] - ref["ke"][-1]) / ref["ke"][-1],
+        "owner_mismatch_fp64_location": m["owner"],
+        "steps_compared": args.err_steps,
+        "phi_l2_rel_per_step": curve(m),
+        "control": {
+            "what": "FP64 form from the same fill rounded to FP32 and back (x, v), against the FP64 form",
+            "phi_l2_rel_after_steps": rel(c["phi"][-1], ref["phi"][-1]),
+            "field_energy_rel_after_steps": abs(fe(c) / fe(ref) - 1.0),
+            "kinetic_energy_rel_after_steps": abs(c["ke"][-1] - ref["ke"][-1]) / ref["ke"][-1],
+            "phi_l2_rel_per_step": curve(c),
+        },
+        "reference": "FP64 form of this build (bit-identical to the FP64 oracle)",
+    }
 
 
 def run_ensemble(args):
@@ -185,21 +266,41 @@ def run_cpu_baseline(args):
     O = load("minipic_oracle", os.path.join(ORACLE, "minipic_oracle.py"))
     import numpy as np
     threads = O.lib().orc_get_threads()
+    # C1 timed in full: Kuhn 8^3 x6, 16 electrons per tet, dt 0.1, 10 steps
+    # from the fill (cold potential, as the reference's run starts)
+    m1 = O.Mesh(8, 8, 8, 6)
+    p1 = O.Particles(m1.ne * 16).fill(m1, 16, 0, DENSITY, temp=2 * 0.1 ** 2, seed_offset=1)
+    bg1 = m1.background(DENSITY)
+    phi1 = np.zeros(m1.nn)
+    S1 = O.charge_scale(m1, p1, bg1)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        O.step(m1, p1, phi1, 0.1, bg=bg1, S=S1)
+    t1 = time.perf_counter() - t0
+    c1 = dict(value=p1.n * 10 / t1, unit="particle-steps/s", cores=threads, kind="port",
+              sample="the whole Kuhn 8^3 x6, 16/tet configuration (%d particles, dt 0.1, 10 steps from the fill), "
+                     "%.2f s" % (p1.n, t1))
+    # the GPU workload: one untimed solve after the fill (the GPU run's warm
+    # start), then --cpu-steps timed steps
     n = args.nx
     m = O.Mesh(n, n, n, 6)
     p = O.Particles(m.ne * args.nppe).fill(m, args.nppe, 0, DENSITY, temp=2 * 0.1 ** 2, seed_offset=1, pert_amp=0.05)
     bg = m.background(DENSITY)
     phi = np.zeros(m.nn)
     S = O.charge_scale(m, p, bg)
+    tw = time.perf_counter()
+    O.cg(m, O.deposit(m, p, bg, S), phi, 1e-10, 1000)
+    tw = time.perf_counter() - tw
     t0 = time.perf_counter()
     steps = args.cpu_steps
     for _ in range(steps):
         O.step(m, p, phi, args.dt, bg=bg, S=S)
     dt = time.perf_counter() - t0
     return dict(value=p.n * steps / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample="%d step(s) of the full %d^3 x6, %d/tet workload (%d particles) from a cold potential, "
-                       "%.1f s; particle phases on %d OpenMP threads, CG single-threaded as the reference's host solve"
-                       % (steps, n, args.nppe, p.n, dt, threads))
+                sample="%d step(s) of the full %d^3 x6, %d/tet workload (%d particles) warm-started like the GPU run "
+                       "(an untimed %.1f s solve after the fill), %.1f s; particle phases on %d OpenMP threads, CG "
+                       "single-threaded as the reference's host solve" % (steps, n, args.nppe, p.n, tw, dt, threads),
+                c1_full=c1)
 
 
 def main():
@@ -212,6 +313,7 @@ def main():
     ap.add_argument("--dt", type=float, default=0.05)
     ap.add_argument("--sort-period", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--err-steps", type=int, default=25, help="untimed steps of the error study (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=10, help="timed steps of the ensemble (batching form); 0 = skip")
     ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg and print its JSON object")
@@ -231,20 +333,9 @@ def main():
         copy_calibration()
     K = args.steps
     sm, sd = summary(gm, K, PUSH_BYTES_MIXED), summary(gd, K, PUSH_BYTES_FP64)
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    ke_m, ke_d = gm["stats"][-1]["kinetic_energy"], gd["stats"][-1]["kinetic_energy"]
-    errors = {
-        "phi_l2_rel_same_particles": rel(gm["phi_fill"], gd["phi_fill"]),
-        "phi_l2_rel_after_steps": rel(gm["phi_end"], gd["phi_end"]),
-        # field energy 0.5 phi^T K phi = 0.5 phi . rhs at the solution
-        "field_energy_rel_after_steps": abs(float(np.dot(gm["phi_end"], gm["rhs_end"]) /
-                                                  np.dot(gd["phi_end"], gd["rhs_end"])) - 1.0),
-        "kinetic_energy_rel_after_steps": abs(ke_m - ke_d) / ke_d,
-        "owner_mismatch_fp64_location": gm["owner_mismatch"],
-        "relocated_last_step": gm["stats"][-1]["relocated"],
-        "steps_compared": args.warmup + K,
-        "reference": "FP64 form of this build (bit-identical to the FP64 oracle)",
-    }
+    errors = run_error_curves(args) if args.err_steps > 0 else {}
+    errors["owner_mismatch_fp64_location_timed_run"] = gm["owner_mismatch"]
+    errors["relocated_last_step"] = gm["stats"][-1]["relocated"]
     ens = run_ensemble(args) if args.ens_steps > 0 else None
     cpu = None
     if not args.no_cpu:
@@ -272,14 +363,21 @@ def main():
             "form": "mixed", "tets": gm["ne"], "nodes": gm["nn"], "nnz": gm["nnz"], "particles": gm["npart"],
             "cg_iters_mean": sm["cg_iters_mean"], "cg_ms_per_step": sm["cg_ms_per_step"],
             "push_ms": sm["push_ms"], "sort_ms_per_step": sm["sort_ms_per_step"],
+            "sort_ms_per_reorder": sm["sort_ms_per_reorder"], "field_ms_per_step": sm["field_ms_per_step"],
+            "timed_steps_of_config": "%d timed steps (+%d warmup) of the configuration's 100; a rate" % (
+                K, args.warmup),
         },
         "fp64": {"value": sd["value"], "ms_per_step": sd["ms_per_step"], "cg_iters_mean": sd["cg_iters_mean"],
                  "cg_ms_per_step": sd["cg_ms_per_step"], "push_ms": sd["push_ms"],
-                 "sort_ms_per_step": sd["sort_ms_per_step"],
+                 "sort_ms_per_step": sd["sort_ms_per_step"], "sort_ms_per_reorder": sd["sort_ms_per_reorder"],
+                 "field_ms_per_step": sd["field_ms_per_step"],
+                 "cg_roofline": cg_roofline(sd, gd["nn"], gd["nnz"], 8, "k_cg_k1 + k_cg_k2"),
                  "push_roofline": {"kernel": "k_push_fused", "achieved": sd["push_gbs"], "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": sd["push_gbs"] / HBM_PEAK_GBS,
                                    "bytes_per_particle": PUSH_BYTES_FP64,
                                    "traffic": measured_traffic("k_push_fused")}},
+        "cg_roofline": cg_roofline(sm, gm["nn"], gm["nnz"], 4, "k_ir_cg (FP32 inner) + k_ir_resid/k_ir_start/"
+                                                                "k_ir_update (FP64 refinement)"),
         "mixed_over_fp64": sm["value"] / sd["value"],
         "errors_vs_fp64": errors,
         "roofline": {"bound": "hbm", "kernel": "k_push32", "achieved": sm["push_gbs"], "peak": HBM_PEAK_GBS,

@@ -615,9 +615,12 @@ class PIC {
     sort_timer_ += last_.ms_sort * 1e-3;
     solve_timer_ += last_.ms_solve * 1e-3;
     KE_timer_ += last_.ms_ke * 1e-3;
+    weight_field_timer_ += last_.ms_field * 1e-3;
     if (verbose_) std::cout << "Total KE is " << last_.kinetic_energy << std::endl;  // PIC.cpp:262-263
   }
-  // PIC::output_timers (PIC.hpp:85-88).  The gather is fused into the move.
+  // PIC::output_timers (PIC.hpp:85-88).  "Weight EField" is the per-element part
+  // of the gather (potential or field per element); the per-particle gather is
+  // fused into the move.
   void output_timers() {
     std::cout << "Timers are:\nMove " << move_timer_ << " Weight charge " << weight_charge_timer_ << ", Sorting "
               << sort_timer_ << ", Solve " << solve_timer_ << ", Weight EField " << weight_field_timer_

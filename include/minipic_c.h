@@ -116,7 +116,11 @@ typedef struct {
   mpic_move_stats move;
   int64_t num_particles;   /* after the step */
   int charge_scale_bits;   /* fixed-point scale used for the node charge sum */
-  double ms_deposit, ms_solve, ms_push, ms_sort, ms_ke;  /* GPU time of the phases */
+  double ms_deposit, ms_solve, ms_push, ms_sort, ms_ke;  /* GPU time of the phases; ms_sort is the
+                                                            reordering done in this step (0 on steps without one) */
+  double ms_field;         /* the per-element part of weight_Efield (PIC.cpp:162-189): potential at the
+                              element's nodes (FP64 form) or the element's field (precision form); the
+                              per-particle gather runs inside the push (ms_push) */
 } mpic_step_stats;
 
 /* ---- configuration -------------------------------------------------------- */

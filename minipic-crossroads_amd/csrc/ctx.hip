@@ -920,6 +920,7 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   if (c->state32) return mixed_time_step(c, dt, out);
   int rc;
   mpic_step_stats st{};
+  c->ms[T_SORT] = 0;  // set again below if this step reorders
   // 1. charge of the current positions (PIC.cpp:250); a pending ranking does not
   //    matter here (the sums are order-free)
   tstart(c, T_DEPOSIT);
@@ -937,7 +938,9 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   }
   tstop(c, T_SOLVE);
   if (rc) return rc;
+  tstart(c, T_FIELD);
   HIPCHK(c, launch_elem_phi(c->m, c->phi, c->stream));
+  tstop(c, T_FIELD);
   // 3. fused gather + kick + walk + deposit of the new positions + KE (PIC.cpp:252-258);
   //    with a pending ranking the push reads through it and writes element order
   if (!(c->ke_max_prev > 0) && c->n > 0) {
@@ -1016,12 +1019,14 @@ int mpic_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   c->ms[T_PUSH] = telapsed(c, T_PUSH);
   c->ms[T_PUSH_KERNEL] = telapsed(c, T_PUSH_KERNEL);
   c->ms[T_KE] = telapsed(c, T_KE);
+  c->ms[T_FIELD] = telapsed(c, T_FIELD);
   if (c->sort_timed) {
     c->ms[T_SORT] = telapsed(c, T_SORT);
     c->sort_timed = false;
   }
   st.num_particles = c->perm_pending ? c->perm_n : c->n;
   st.charge_scale_bits = c->S;
+  st.ms_field = c->ms[T_FIELD];
   st.ms_deposit = c->ms[T_DEPOSIT];
   st.ms_solve = c->ms[T_SOLVE];
   st.ms_push = c->ms[T_PUSH];

@@ -50,6 +50,7 @@ void record_times(mpic_ctx* c) {
   c->ms[T_PUSH] = telapsed(c, T_PUSH);
   c->ms[T_PUSH_KERNEL] = telapsed(c, T_PUSH_KERNEL);
   c->ms[T_KE] = telapsed(c, T_KE);
+  c->ms[T_FIELD] = telapsed(c, T_FIELD);
   if (c->sort_timed) {
     c->ms[T_SORT] = telapsed(c, T_SORT);
     c->sort_timed = false;
@@ -274,6 +275,7 @@ int mixed_check_owner(mpic_ctx* c, int64_t* bad) {
 int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   int rc;
   mpic_step_stats st{};
+  c->ms[T_SORT] = 0;  // set again below if this step reorders
   // 1. charge of the current positions
   tstart(c, T_DEPOSIT);
   if ((rc = mixed_weight_charge(c))) return rc;
@@ -315,7 +317,9 @@ int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
     a.count = c->sb.count;
   }
   tstart(c, T_PUSH);
+  tstart(c, T_FIELD);
   HIPCHK(c, launch_field32(c->m, c->phi, c->m32.E, c->stream));
+  tstop(c, T_FIELD);
   tstart(c, T_PUSH_KERNEL);
   HIPCHK(c, launch_push32(a, c->stream));
   tstop(c, T_PUSH_KERNEL);
@@ -373,6 +377,7 @@ int mixed_time_step(mpic_ctx* c, double dt, mpic_step_stats* out) {
   st.ms_push = c->ms[T_PUSH];
   st.ms_sort = c->ms[T_SORT];
   st.ms_ke = c->ms[T_KE];
+  st.ms_field = c->ms[T_FIELD];
   if (out) *out = st;
   // The same contract as the FP64 form: the step completes (a particle stopped
   // by the crossing cap stays where the cap found it, inside its element after

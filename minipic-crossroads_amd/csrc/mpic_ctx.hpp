@@ -10,7 +10,7 @@
 #include "mpic_internal.hpp"
 
 namespace mpic {
-enum Timer { T_DEPOSIT, T_SOLVE, T_PUSH, T_SORT, T_KE, T_PUSH_KERNEL, T_COUNT };
+enum Timer { T_DEPOSIT, T_SOLVE, T_PUSH, T_SORT, T_KE, T_PUSH_KERNEL, T_FIELD, T_COUNT };
 std::string& global_error();
 }  // namespace mpic
 

@@ -76,7 +76,7 @@ class StepStats(C.Structure):
     _fields_ = [("cg_iters", C.c_int), ("cg_relres", C.c_double), ("kinetic_energy", C.c_double),
                 ("move", MoveStats), ("num_particles", C.c_int64), ("charge_scale_bits", C.c_int),
                 ("ms_deposit", C.c_double), ("ms_solve", C.c_double), ("ms_push", C.c_double),
-                ("ms_sort", C.c_double), ("ms_ke", C.c_double)]
+                ("ms_sort", C.c_double), ("ms_ke", C.c_double), ("ms_field", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "move"}
