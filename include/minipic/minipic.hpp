@@ -13,17 +13,21 @@
 //   PIC            PIC.hpp:54-106             weight_charge / weight_Efield / time_step / timers
 // Copies of a handle share one problem (as Kokkos views share memory).  Errors
 // throw minipic::Error carrying the library status and message, where the
-// reference asserts.  The reference's MPI communicator arguments are absent:
-// one problem runs on one GPU.
+// reference asserts.  One problem runs on one GPU: where the reference takes a
+// Teuchos communicator, any communicator object is accepted and the single
+// rank is used (minipic::default_comm() gives one with getRank()/getSize()).
 #ifndef MINIPIC_MINIPIC_HPP_
 #define MINIPIC_MINIPIC_HPP_
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -58,6 +62,16 @@ class ConstLenVector {
 typedef ConstLenVector<FLOAT, 3> VECTOR;
 
 namespace minipic {
+
+// The communicator of a one-GPU run: rank 0 of 1 (the calls of
+// Teuchos::Comm<int> that the reference's driver makes on it).
+struct SingleRankComm {
+  int getRank() const { return 0; }
+  int getSize() const { return 1; }
+  void barrier() const {}
+};
+// Teuchos::DefaultComm<int>::getComm() of a one-GPU run (an RCP-like handle).
+inline std::shared_ptr<const SingleRankComm> default_comm() { return std::make_shared<const SingleRankComm>(); }
 
 class Error : public std::runtime_error {
  public:
@@ -449,6 +463,14 @@ class DataWarehouse {
   // "nx ny nz mapping"; [-1,1]^3 split into 5 tets per hex.  periodic = true joins
   // the x = -1 and x = +1 faces (the two-stream problem's mesh; nx even): its
   // stiffness matrix is not symmetric, so the solver becomes BiCGStab.
+  // data_warehouse.hpp:66, create_mesh(std::string&, RCP<Comm>, bool): main.cpp:91
+  // calls create_mesh(meshfile, comm, periodic).  The communicator is accepted
+  // and unused (one rank).
+  template <class Comm>
+  void create_mesh(const std::string& in_name, const Comm& comm, bool periodic = false) {
+    (void)comm;
+    create_mesh(in_name, periodic);
+  }
   void create_mesh(const std::string& in_name, bool periodic = false) {
     std::ifstream in(in_name.c_str());
     int nx = 0, ny = 0, nz = 0, mapping = 0;
@@ -511,20 +533,108 @@ class DataWarehouse {
     if (c_->ctx) minipic::detail::check(mpic_set_kill_on_wall(c_->ctx, flag), c_->ctx, "set_kill");
   }
   bool kill_particles_on_wall_impact() { return kill_particles_on_wall_impact_; }
-  // dump_mesh_data (mesh.cpp:794-910): node coordinates and element node lists as text.
-  void dump_mesh_data() {
+  // dump_mesh_data (data_warehouse.hpp, Mesh::dump_data mesh.cpp:794-910): the
+  // rank-0 files of the reference, in its line formats (default stream
+  // precision, a space after every value):
+  //   nodes-000.dat  gid x y z          elems-000.dat  gid n0 n1 n2 n3
+  //   faces-000.dat  gid a b c          edges-000.dat  gid a b
+  //   owned-faces-000.dat, owned-edges-000.dat, boundary-faces-000.dat,
+  //   boundary-edges-000.dat: one gid per line.
+  // Faces and edges are numbered as Mesh::Mesh numbers them (mesh.cpp:233-315):
+  // the sorted node triples (pairs) of every element's faces (edges), in
+  // lexicographic order; on a periodic mesh a face lying on x = hi is keyed by
+  // its x = lo image (Mesh::periodic_face, mesh.cpp:62-76).  A face (edge) is
+  // on the boundary when its centroid is within 1 - 0.99999999 of a wall in
+  // the box's [-1, 1] coordinates (mesh.cpp:604-631).  One rank owns all.
+  void dump_mesh_data(const std::string& dir = ".") {
     if (!c_->mesh) return;
-    std::ofstream nodes("mesh_nodes.0"), elems("mesh_elems.0");
-    nodes.precision(17);
-    for (LO i = 0; i < c_->desc.num_nodes; ++i)
-      nodes << i << " " << c_->desc.nodes[(size_t)i * 3] << " " << c_->desc.nodes[(size_t)i * 3 + 1] << " "
-            << c_->desc.nodes[(size_t)i * 3 + 2] << "\n";
-    for (LO e = 0; e < c_->desc.num_elems; ++e) {
-      elems << e;
-      for (int k = 0; k < 4; ++k) elems << " " << c_->desc.elem_nodes[(size_t)e * 4 + k];
-      for (int k = 0; k < 4; ++k) elems << " " << c_->desc.elem_nbrs[(size_t)e * 4 + k];
-      elems << "\n";
+    const mpic_mesh_desc& d = c_->desc;
+    const LO nn = d.num_nodes, ne = d.num_elems;
+    const std::string base = dir.empty() ? std::string(".") : dir;
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = hi[a] = d.nodes[a];
+      for (LO i = 1; i < nn; ++i) {
+        lo[a] = std::min(lo[a], d.nodes[(size_t)i * 3 + a]);
+        hi[a] = std::max(hi[a], d.nodes[(size_t)i * 3 + a]);
+      }
     }
+    auto X = [&](LO n, int a) { return d.nodes[(size_t)n * 3 + a]; };
+    // x = hi node -> its x = lo image (same y, z) on a periodic mesh
+    std::map<std::pair<double, double>, LO> lo_plane;
+    if (d.node_pair)
+      for (LO i = 0; i < nn; ++i)
+        if (X(i, 0) == lo[0]) lo_plane[{X(i, 1), X(i, 2)}] = i;
+    std::map<std::array<LO, 3>, GO> faces;
+    std::map<std::array<LO, 2>, GO> edges;
+    static const int fv[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
+    for (LO e = 0; e < ne; ++e) {
+      const int32_t* en = d.elem_nodes + (size_t)e * 4;
+      for (int f = 0; f < 4; ++f) {
+        std::array<LO, 3> k{en[fv[f][0]], en[fv[f][1]], en[fv[f][2]]};
+        if (d.node_pair && X(k[0], 0) == hi[0] && X(k[1], 0) == hi[0] && X(k[2], 0) == hi[0])
+          for (LO& n : k) n = lo_plane.at({X(n, 1), X(n, 2)});
+        std::sort(k.begin(), k.end());
+        faces.emplace(k, 0);
+      }
+      for (int j = 0; j < 3; ++j)
+        for (int k = j + 1; k < 4; ++k) {
+          std::array<LO, 2> ed{std::min(en[j], en[k]), std::max(en[j], en[k])};
+          edges.emplace(ed, 0);
+        }
+    }
+    GO g = 0;
+    for (auto& f : faces) f.second = g++;
+    g = 0;
+    for (auto& ed : edges) ed.second = g++;
+    const double almost_one = 0.99999999;  // mesh.cpp:367
+    auto on_wall = [&](const double* c) {
+      for (int a = 0; a < 3; ++a) {
+        const double s = (2.0 * c[a] - (lo[a] + hi[a])) / (hi[a] - lo[a]);
+        if (s <= -almost_one || s >= almost_one) return true;
+      }
+      return false;
+    };
+    std::ofstream out;
+    out.open(base + "/nodes-000.dat");
+    for (LO i = 0; i < nn; ++i) out << i << ' ' << X(i, 0) << ' ' << X(i, 1) << ' ' << X(i, 2) << ' ' << std::endl;
+    out.close();
+    out.open(base + "/elems-000.dat");
+    for (LO e = 0; e < ne; ++e) {
+      out << e << ' ';
+      for (int k = 0; k < 4; ++k) out << d.elem_nodes[(size_t)e * 4 + k] << ' ';
+      out << std::endl;
+    }
+    out.close();
+    std::vector<GO> bface, bedge;
+    out.open(base + "/faces-000.dat");
+    for (const auto& f : faces) {
+      out << f.second << ' ' << f.first[0] << ' ' << f.first[1] << ' ' << f.first[2] << ' ' << std::endl;
+      double c[3];
+      for (int a = 0; a < 3; ++a) c[a] = (X(f.first[0], a) + X(f.first[1], a) + X(f.first[2], a)) / 3;
+      if (on_wall(c)) bface.push_back(f.second);
+    }
+    out.close();
+    out.open(base + "/edges-000.dat");
+    for (const auto& ed : edges) {
+      out << ed.second << ' ' << ed.first[0] << ' ' << ed.first[1] << ' ' << std::endl;
+      double c[3];
+      for (int a = 0; a < 3; ++a) c[a] = (X(ed.first[0], a) + X(ed.first[1], a)) / 2;
+      if (on_wall(c)) bedge.push_back(ed.second);
+    }
+    out.close();
+    out.open(base + "/owned-faces-000.dat");
+    for (GO i = 0; i < (GO)faces.size(); ++i) out << i << std::endl;
+    out.close();
+    out.open(base + "/owned-edges-000.dat");
+    for (GO i = 0; i < (GO)edges.size(); ++i) out << i << std::endl;
+    out.close();
+    out.open(base + "/boundary-faces-000.dat");
+    for (GO i : bface) out << i << std::endl;
+    out.close();
+    out.open(base + "/boundary-edges-000.dat");
+    for (GO i : bedge) out << i << std::endl;
+    out.close();
   }
 
   // The library context behind the classes (for calls the classes do not wrap).

@@ -189,6 +189,37 @@ int mpic_num_particles(mpic_ctx* ctx, int64_t* n);                    /* Particl
 int mpic_fill(mpic_ctx* ctx, int nppe, int type, double density, const double* vel, double temp,
               uint64_t seed_offset, double pert_amp);
 
+/* ---- device access to the particle list (ParticleList's functor API,
+ * particle_list.hpp:54-213) ------------------------------------------------------
+ * For HIP kernels written against the reference's ParticleList accessors and
+ * team-collective mutators: include/minipic/particle_list_view.hpp builds
+ * minipic::ParticleListView from this descriptor.  All pointers are device
+ * memory owned by the context. */
+typedef struct {
+  void* records;                 /* [capacity] 96-B particle records (minipic::ParticleRecord) */
+  int32_t* elem_copy;            /* [capacity] element ids as the counting sort reads them */
+  double* E;                     /* [3][capacity] ParticleList::E (written by mpic_weight_efield) */
+  double* v_ref;                 /* [3][capacity] ParticleList::v_ref (written by mpic_accel) */
+  double* time_remaining;        /* [capacity]    ParticleList::time_remaining */
+  unsigned long long* counters;  /* [4]: particles, deleted, migrate entries, dropped appends */
+  int64_t* migrate_index;        /* [migrate_capacity] migrate list: particle index ... */
+  int32_t* migrate_proc;         /* [migrate_capacity] ... and destination rank */
+  int64_t capacity, migrate_capacity;
+} mpic_particle_view;
+
+/* Opens the device list for user kernels (FP64 form): applies a pending
+ * reordering, so records [0, n) are the live particles, and resets the
+ * counters (particles = n).  Kernels that use the view run on mpic_stream(). */
+int mpic_particles_view(mpic_ctx* ctx, mpic_particle_view* view);
+/* Closes it (the reference's cleanup_list, particle_list.cpp:148-191): waits
+ * for the stream, takes the particle count from the device counter, refreshes
+ * the element copy from the records and compacts deleted particles; reports
+ * the migrate entries (one GPU: nothing is sent).  MPIC_ERR_CAPACITY if an
+ * append was dropped for lack of capacity (the rest is committed). */
+int mpic_particles_view_commit(mpic_ctx* ctx, int64_t* num_particles, int64_t* num_migrate);
+/* The context's HIP stream (hipStream_t), on which every call is ordered. */
+int mpic_stream(mpic_ctx* ctx, void** stream);
+
 /* ---- phases of PIC::time_step (PIC.cpp:245-267) ----------------------------- */
 int mpic_weight_charge(mpic_ctx* ctx);                                 /* PIC::weight_charge, PIC.cpp:139-160 */
 /* ES::solve, ES.cpp:216-231, with cfg.solver: CG_solve.hpp:54-114 (warm start) or

@@ -359,6 +359,7 @@ int mpic_batch_fill(mpic_batch* b, const mpic_batch_fill_desc* f) {
       a.lo0 = b->lo0[p];
       a.mass = c->cfg.species_m[0];
       a.seed_offset = f[p].seed_offset + (uint64_t)half;
+      a.view = particle_view(c);
       BCHK(b, launch_fill(a, c->stream));
     }
   }

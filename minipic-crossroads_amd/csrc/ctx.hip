@@ -78,8 +78,25 @@ PushArgs push_args(mpic_ctx* c) {
   a.E = c->E;
   a.vr = c->vr;
   a.species = c->species;
+  a.view = particle_view(c);
   return a;
 }
+
+minipic::ParticleListView particle_view(mpic_ctx* c) {
+  minipic::ParticleListView v{};
+  v.rec = reinterpret_cast<minipic::ParticleRecord*>(c->P[c->cur].r);
+  v.elem_copy = c->P[c->cur].elem;
+  v.E_ = c->E;
+  v.v_ref_ = c->vr;
+  v.time_remaining_ = c->tr;
+  v.counters = c->view_counters;
+  v.migrate_index = c->migrate_index;
+  v.migrate_proc = c->migrate_proc;
+  v.capacity = c->cap;
+  v.migrate_capacity = MIGRATE_CAP;
+  return v;
+}
+
 
 // Reduce the per-workgroup partials of the last particle launch; synchronises.
 // Waits for the stream by polling it: a blocking wait adds tens of
@@ -157,8 +174,9 @@ int free_particles(mpic_ctx* c) {
     }
   if (c->E) (void)hipFree(c->E);
   if (c->vr) (void)hipFree(c->vr);
+  if (c->tr) (void)hipFree(c->tr);
   if (c->perm) (void)hipFree(c->perm);
-  c->E = c->vr = nullptr;
+  c->E = c->vr = c->tr = nullptr;
   c->perm = nullptr;
   c->perm_pending = false;
   mixed_free_particles(c);
@@ -588,7 +606,8 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       (rc = dalloc(c, &c->cg.st, 1)) || (rc = dalloc(c, &c->sb.count, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.start, (size_t)ne + 2)) || (rc = dalloc(c, &c->sb.cursor, (size_t)ne + 2)) ||
       (rc = dalloc(c, &c->sb.blocksum, (size_t)ne / 2048 + 64)) || (rc = dalloc(c, &c->finish, 1)) ||
-      (rc = dalloc(c, &c->partial, PARTIAL_SLOTS)))
+      (rc = dalloc(c, &c->partial, PARTIAL_SLOTS)) || (rc = dalloc(c, &c->view_counters, 4)) ||
+      (rc = dalloc(c, &c->migrate_index, (size_t)MIGRATE_CAP)) || (rc = dalloc(c, &c->migrate_proc, (size_t)MIGRATE_CAP)))
     return bail(rc);
   c->cg.nchunks = nchunks;
   if (hipHostMalloc((void**)&c->cg_h, sizeof(CGState)) != hipSuccess ||
@@ -801,6 +820,7 @@ int mpic_fill(mpic_ctx* c, int nppe, int type, double density, const double* vel
   a.pert_k = (2.0 * M_PI) / (c->m.hi[0] - c->m.lo[0]);
   a.mass = c->cfg.species_m[type];
   a.seed_offset = seed_offset;
+  a.view = particle_view(c);
   HIPCHK(c, launch_fill(a, c->stream));
   int rc;
   if (c->state32 && (rc = mixed_import(c, c->n, add))) return rc;
@@ -1091,6 +1111,57 @@ int mpic_check_ownership(mpic_ctx* c, int64_t* mismatches) {
       return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
   }
   return mixed_check_owner(c, mismatches);
+}
+
+int mpic_particles_view(mpic_ctx* c, mpic_particle_view* out) {
+  if (!c || !out) return fail(c, MPIC_ERR_ARG, "mpic_particles_view: bad arguments");
+  if (c->cap == 0) return fail(c, MPIC_ERR_STATE, "mpic_particles_view: no particle list");
+  if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_particles_view: " FP64_ONLY);
+  int rc;
+  if ((rc = materialize(c)) || (rc = ensure_transients(c))) return rc;
+  if (!c->tr) HIPCHK(c, hipMalloc((void**)&c->tr, (size_t)c->cap * sizeof(double)));
+  HIPCHK(c, launch_view_reset(c->view_counters, c->n, c->stream));
+  const minipic::ParticleListView v = particle_view(c);
+  out->records = v.rec;
+  out->elem_copy = v.elem_copy;
+  out->E = v.E_;
+  out->v_ref = v.v_ref_;
+  out->time_remaining = v.time_remaining_;
+  out->counters = v.counters;
+  out->migrate_index = v.migrate_index;
+  out->migrate_proc = v.migrate_proc;
+  out->capacity = v.capacity;
+  out->migrate_capacity = v.migrate_capacity;
+  return MPIC_OK;
+}
+
+int mpic_particles_view_commit(mpic_ctx* c, int64_t* num_particles, int64_t* num_migrate) {
+  if (!c) return MPIC_ERR_ARG;
+  if (c->state32) return fail(c, MPIC_ERR_STATE, "mpic_particles_view_commit: " FP64_ONLY);
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(cnt, c->view_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  c->perm_pending = false;
+  c->n = (int64_t)std::min<unsigned long long>(cnt[0], (unsigned long long)c->cap);
+  // the element copy from the records (kernels may have written ielement)
+  HIPCHK(c, launch_elem_copy(c->P[c->cur], c->n, c->stream));
+  int rc;
+  if ((rc = invalidate_deposit(c))) return rc;
+  c->ke_max_prev = 0;
+  if (cnt[1] > 0 && (rc = do_sort(c, nullptr))) return rc;  // cleanup_list: compact the dead
+  if ((rc = update_charge_scale(c))) return rc;
+  if (num_particles) *num_particles = c->n;
+  if (num_migrate) *num_migrate = (int64_t)std::min<unsigned long long>(cnt[2], (unsigned long long)MIGRATE_CAP);
+  if (cnt[3] > 0)
+    return fail(c, MPIC_ERR_CAPACITY, "ParticleList::push_back: " + std::to_string(cnt[3]) +
+                                          " particle(s) beyond the capacity were not added");
+  return MPIC_OK;
+}
+
+int mpic_stream(mpic_ctx* c, void** stream) {
+  if (!c || !stream) return MPIC_ERR_ARG;
+  *stream = (void*)c->stream;
+  return MPIC_OK;
 }
 
 int mpic_kernel_time(mpic_ctx* c, int which, double* ms) {

@@ -228,6 +228,7 @@ struct KArgs {
   double* E;
   double* vr;
   const double* sp;  // [48]: q[16], m[16], q/m[16]
+  minipic::ParticleListView view;  // k_move: wall kills go through add_particles_to_migrate_list
 };
 
 __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
@@ -335,34 +336,51 @@ __global__ __launch_bounds__(BLOCK) void k_accel(KArgs a) {
 }
 
 // MoveParticle: the tet walk from the stored v and v_ref (no deposit).
+// MoveParticle::operator() (particle_move.hpp:103-124): walk each particle, then
+// the team hands its wall kills (KILL_ONLY) to the team-collective
+// ParticleList::add_particles_to_migrate_list, which marks them dead; the host
+// then compacts the list (cleanup_list).
 __global__ __launch_bounds__(BLOCK) void k_move(KArgs a) {
+  __shared__ long team_id[BLOCK];
+  __shared__ int team_proc[BLOCK];
+  __shared__ int scratch[MPIC_TEAM_SCRATCH];
+  const minipic::member_type dev(scratch);
   const int64_t cap = a.P.cap;
   WalkCount wc{};
   long long killed = 0;
   for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < a.n; base += (int64_t)gridDim.x * BLOCK) {
     const int64_t p = base + threadIdx.x;
-    if (p >= a.n) continue;
-    Rec64 rc = load_rec(a.P.r, p);
-    int e = rc.elem;
-    if (e < 0) continue;
-    double X[3], R[3], V[3], VR[3];
-    for (int d = 0; d < 3; ++d) {
-      X[d] = rc.x[d];
-      R[d] = rc.xr[d];
-      V[d] = rc.v[d];
-      VR[d] = a.vr[d * cap + p];
+    int proc = minipic::ParticleListView::DO_NOTHING;
+    if (p < a.n) {
+      Rec64 rc = load_rec(a.P.r, p);
+      int e = rc.elem;
+      if (e >= 0) {
+        double X[3], R[3], V[3], VR[3];
+        for (int d = 0; d < 3; ++d) {
+          X[d] = rc.x[d];
+          R[d] = rc.xr[d];
+          V[d] = rc.v[d];
+          VR[d] = a.vr[d * cap + p];
+        }
+        ElemRec g = load_erec(a.m.erec, e);
+        const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);
+        for (int d = 0; d < 3; ++d) {
+          rc.x[d] = X[d];
+          rc.xr[d] = R[d];
+          rc.v[d] = V[d];
+        }
+        rc.elem = e;
+        store_rec(a.P.r, p, rc);
+        a.P.elem[p] = e;
+        if (!alive) {
+          proc = minipic::ParticleListView::KILL_ONLY;
+          killed++;
+        }
+      }
     }
-    ElemRec g = load_erec(a.m.erec, e);
-    const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);
-    for (int d = 0; d < 3; ++d) {
-      rc.x[d] = X[d];
-      rc.xr[d] = R[d];
-      rc.v[d] = V[d];
-    }
-    rc.elem = alive ? e : -1;
-    store_rec(a.P.r, p, rc);
-    a.P.elem[p] = rc.elem;
-    killed += alive ? 0 : 1;
+    team_id[threadIdx.x] = (long)p;
+    team_proc[threadIdx.x] = proc;
+    a.view.add_particles_to_migrate_list(dev, team_id, team_proc, BLOCK);
   }
   block_partial(a.partial, fx128{0, 0}, 0ull, wc, killed);
 }
@@ -489,7 +507,7 @@ __global__ __launch_bounds__(FINISH_T) void k_push_finish(const PushPartial* __r
 
 KArgs kargs(const PushArgs& a) {
   return KArgs{a.m, a.P, a.perm ? a.Q : a.P, a.perm, a.count, a.n, a.phi, a.dt, a.scale, a.kill, a.S_charge,
-               a.S_ke, a.acc, a.partial, a.E, a.vr, a.species};
+               a.S_ke, a.acc, a.partial, a.E, a.vr, a.species, a.view};
 }
 
 }  // namespace

@@ -24,6 +24,7 @@
 //   --quiet 1         no per-step KE line
 // Prints the reference's "Fill time", per-step "Total KE is", "Move time ...
 // updates/second" and timer lines.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -34,6 +35,8 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "minipic.hpp"
 
@@ -103,7 +106,10 @@ int run(const Options& o) {
       data.create_mesh_brick(nx, ny, nz, (int)o.integer("split", 5), lo, hi, o.num("jitter", 0.0),
                              (uint64_t)o.integer("jitter-seed", 0), 0);
   } else {
-    data.create_mesh(o.get("mesh", "brick.txt"), two_stream);  // main.cpp:89-92
+    auto comm = minipic::default_comm();  // main.cpp:70 (one rank)
+    std::string meshfile = o.get("mesh", "brick.txt");
+    const bool periodic = two_stream;
+    data.create_mesh(meshfile, comm, periodic);  // main.cpp:89-92
   }
   mpic_config& cfg = data.config();
   cfg.form = o.get("form", "fp64") == "mixed" ? MPIC_FORM_MIXED : MPIC_FORM_FP64;
@@ -157,14 +163,23 @@ int run(const Options& o) {
   std::cout.precision(17);
   for (double t = 0; steps_opt >= 0 ? steps < steps_opt : t < t_final; t += dt, ++steps) {
     if (output && (output_cnt++) % output == 0) {  // main.cpp:125-136
+      // The reference writes list positions 0, nppe, 2 nppe, ...; its list
+      // keeps the fill order (the OpenMP path never reorders it), so these are
+      // the particles of fill index 0, nppe, 2 nppe, ...: this build's list is
+      // reordered by element, and the same particles are found by their
+      // identity key (fill index), written in that order.
       char name[256];
       std::snprintf(name, sizeof(name), "output.%f", t);
       std::ofstream out(name);
       ParticleList& pl = data.particles_reference();
       const long nparts = (long)pl.num_particles();
-      for (long i = 0; i < nparts; i += nppe)
-        if (pl.type(i) == 0) {
-          const VECTOR x = pl.x(i), v = pl.v(i);
+      std::vector<std::pair<uint32_t, long>> pick;
+      for (long i = 0; i < nparts; ++i)
+        if (pl.id(i) % (uint32_t)nppe == 0) pick.emplace_back(pl.id(i), i);
+      std::sort(pick.begin(), pick.end());
+      for (const auto& q : pick)
+        if (pl.type(q.second) == 0) {
+          const VECTOR x = pl.x(q.second), v = pl.v(q.second);
           out << x[0] << " " << x[1] << " " << x[2] << " " << v[0] << " " << v[1] << " " << v[2] << std::endl;
         }
     }

@@ -53,6 +53,12 @@ struct mpic_ctx {
   void* pmem[2] = {nullptr, nullptr};
   double* E = nullptr;   // transient [3][cap]
   double* vr = nullptr;  // transient [3][cap]
+  double* tr = nullptr;  // transient [cap]: time_remaining of the device view
+  // device ParticleList view: counters [4] (particles, deleted, migrate
+  // entries, dropped appends) and the migrate list
+  unsigned long long* view_counters = nullptr;
+  int64_t* migrate_index = nullptr;
+  int32_t* migrate_proc = nullptr;
   PushPartial* partial = nullptr;
   PushPartial* finish = nullptr;   // device [1]
   PushPartial* finish_h = nullptr; // pinned host [1]
@@ -157,6 +163,8 @@ int finish_partials(mpic_ctx* c, int parts, PushPartial* out);
 int materialize(mpic_ctx* c);
 int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live);
 int rank_enqueue(mpic_ctx* c, bool by_bucket, bool counts_ready);
+// The device ParticleList view of the FP64 set P[cur] (counters not reset).
+minipic::ParticleListView particle_view(mpic_ctx* c);
 void rank_finish(mpic_ctx* c, bool by_bucket);
 hipError_t wait_stream(hipStream_t s);
 // Buffers and capacities of the persistent solves (once per context).

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "minipic_c.h"
+#include "minipic/particle_list_view.hpp"
 
 namespace mpic {
 
@@ -85,6 +86,17 @@ struct alignas(16) Rec64 {
   uint32_t pad;
 };
 static_assert(sizeof(Rec64) == 96, "FP64 particle record is six 16-B words");
+// The device ParticleList view (include/minipic/particle_list_view.hpp) reads
+// the same records.
+static_assert(offsetof(Rec64, xr) == offsetof(minipic::ParticleRecord, x_ref) &&
+                  offsetof(Rec64, v) == offsetof(minipic::ParticleRecord, v) &&
+                  offsetof(Rec64, w) == offsetof(minipic::ParticleRecord, weight) &&
+                  offsetof(Rec64, elem) == offsetof(minipic::ParticleRecord, ielement) &&
+                  offsetof(Rec64, id) == offsetof(minipic::ParticleRecord, id) &&
+                  offsetof(Rec64, type) == offsetof(minipic::ParticleRecord, type),
+              "Rec64 and minipic::ParticleRecord differ");
+// Entries of the view's migrate list (migrations only arise between ranks).
+constexpr int64_t MIGRATE_CAP = 1 << 16;
 
 struct DevParticles {
   Rec64* r;       // [cap]
@@ -232,6 +244,7 @@ struct PushArgs {
   double* E;               // transient [3][cap] for the split entry points
   double* vr;              // transient [3][cap]
   const double* species;   // device [48]: q[16], m[16], q/m[16]
+  minipic::ParticleListView view;  // the list as the move's kill path sees it (split entry points)
 };
 
 hipError_t launch_push_fused(const PushArgs& a, hipStream_t s);   // gather+kick+walk+deposit+KE
@@ -407,8 +420,15 @@ struct FillArgs {
   int nppe, type;
   double density, vel[3], temp, pert_amp, pert_k, lo0, mass;
   uint64_t seed_offset;
+  minipic::ParticleListView view;  // the list the fill appends to (push_back; its count is set to offset)
 };
+// Appends ecount * nppe particles through a.view.push_back; the view's
+// particle counter is first set to a.offset.
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+// View counters := {n, 0, 0, 0}.
+hipError_t launch_view_reset(unsigned long long* counters, int64_t n, hipStream_t s);
+// P.elem[i] = P.r[i].elem for i < n.
+hipError_t launch_elem_copy(const DevParticles& P, int64_t n, hipStream_t s);
 
 // ---- batching form (k_batch.hip) ----
 struct BatchSolveArgs {

@@ -31,6 +31,7 @@ EXPORTS = [
     "mpic_batch_create", "mpic_batch_destroy", "mpic_batch_fill", "mpic_batch_tune", "mpic_batch_step",
     "mpic_batch_graph", "mpic_batch_problem_stats", "mpic_batch_get_phi", "mpic_batch_get_rhs",
     "mpic_batch_download", "mpic_batch_check_ownership", "mpic_batch_last_error",
+    "mpic_particles_view", "mpic_particles_view_commit", "mpic_stream",
 ]
 
 
@@ -143,6 +144,9 @@ def lib():
             "mpic_batch_download": (i32, [P, i32, P, P]),
             "mpic_batch_check_ownership": (i32, [P, P]),
             "mpic_batch_last_error": (C.c_char_p, [P]),
+            "mpic_particles_view": (i32, [P, P]),
+            "mpic_particles_view_commit": (i32, [P, P, P]),
+            "mpic_stream": (i32, [P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -190,3 +190,109 @@ def test_driver_two_stream_matches_oracle(oracle):
         O.push(om, p, phi, 0.2)
         ke = O.ke(p)
         assert abs(kes[s] - ke) <= 1e-13 * abs(ke), (s, kes[s], ke)
+
+
+DUMP = os.path.join(ROOT, "tests", "cpp", "mesh_dump_check")
+
+
+def expected_dump(om, periodic):
+    """Mesh::dump_data's files (mesh.cpp:794-910) computed here from the oracle
+    mesh: faces / edges keyed by sorted node tuples in lexicographic order
+    (mesh.cpp:233-315, with Mesh::periodic_face, mesh.cpp:62-76), boundary by
+    the centroid test of mesh.cpp:604-631, %g formatting (default precision)."""
+    X, en = om.nodes, om.elem_nodes
+    lo, hi = X.min(0), X.max(0)
+    lo_plane = {(x[1], x[2]): i for i, x in enumerate(X) if x[0] == lo[0]}
+    faces, edges = set(), set()
+    for e in range(om.ne):
+        for f in ((0, 1, 3), (1, 2, 3), (0, 3, 2), (0, 2, 1)):
+            k = [int(en[e][j]) for j in f]
+            if periodic and all(X[n][0] == hi[0] for n in k):
+                k = [lo_plane[(X[n][1], X[n][2])] for n in k]
+            faces.add(tuple(sorted(k)))
+        for j in range(3):
+            for m in range(j + 1, 4):
+                edges.add(tuple(sorted((int(en[e][j]), int(en[e][m])))))
+    faces, edges = sorted(faces), sorted(edges)
+
+    def wall(c):
+        s = (2 * c - (lo + hi)) / (hi - lo)
+        return bool(np.any(s <= -0.99999999) or np.any(s >= 0.99999999))
+
+    g = "%g"
+    out = {
+        "nodes-000.dat": "".join("%d %s %s %s \n" % (i, g % x[0], g % x[1], g % x[2]) for i, x in enumerate(X)),
+        "elems-000.dat": "".join("%d %d %d %d %d \n" % (e, *en[e]) for e in range(om.ne)),
+        "faces-000.dat": "".join("%d %d %d %d \n" % (i, *f) for i, f in enumerate(faces)),
+        "edges-000.dat": "".join("%d %d %d \n" % (i, *ed) for i, ed in enumerate(edges)),
+        "owned-faces-000.dat": "".join("%d\n" % i for i in range(len(faces))),
+        "owned-edges-000.dat": "".join("%d\n" % i for i in range(len(edges))),
+        "boundary-faces-000.dat": "".join("%d\n" % i for i, f in enumerate(faces)
+                                          if wall((X[f[0]] + X[f[1]] + X[f[2]]) / 3)),
+        "boundary-edges-000.dat": "".join("%d\n" % i for i, ed in enumerate(edges) if wall((X[ed[0]] + X[ed[1]]) / 2)),
+    }
+    return out, len(faces), len(edges)
+
+
+@pytest.mark.parametrize("nx,ny,nz,mapping,periodic", [(2, 3, 2, 0, False), (3, 2, 2, 1, False), (2, 2, 3, 3, False),
+                                                       (4, 2, 3, 0, True)])
+def test_mesh_dump_reference_files(oracle, tmp_path, nx, ny, nz, mapping, periodic):
+    """main.cpp:88-92's setup (create_mesh(meshfile, comm, periodic)) then the
+    dump, host code only.  Pinned by an independent enumeration, by the
+    reference's own boundary counts (mesh.cpp:641-647) and by Euler's formula
+    V - E + F - T = 1 of a tetrahedral ball."""
+    assert os.path.exists(DUMP), "run build() first"
+    brick = tmp_path / "brick.txt"
+    brick.write_text("%d %d %d %d\n" % (nx, ny, nz, mapping))
+    r = subprocess.run([DUMP, str(brick), "1" if periodic else "0", str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("rank 0 of 1")
+    om = oracle.Mesh(nx, ny, nz, 5, lo=(-1, -1, -1), hi=(1, 1, 1), mapping=mapping, periodic=periodic)
+    want, nf, ned = expected_dump(om, periodic)
+    for name, text in want.items():
+        assert (tmp_path / name).read_text() == text, name
+    if not periodic:
+        assert 1 == om.nn - ned + nf - om.ne
+        nb_f = len(want["boundary-faces-000.dat"].split())
+        nb_e = len(want["boundary-edges-000.dat"].split())
+        assert nb_f == 2 * (2 * nx * ny + 2 * nx * nz + 2 * ny * nz)
+        assert nb_e == (2 * (5 * nx * ny - nx * (ny - 1) - ny * (nx - 1) + 5 * nx * nz - nx * (nz - 1) - nz * (nx - 1)
+                             + 5 * ny * nz - ny * (nz - 1) - nz * (ny - 1)) - 4 * (nx + ny + nz))
+    else:
+        # the 2 ny nz wall triangles on x = hi fold onto their x = lo images
+        # (faces only: edges are not joined, mesh.cpp:252), and count once as
+        # boundary faces (centroid on x = lo)
+        assert nf == 1 - om.nn + ned + om.ne - 2 * ny * nz
+        nb_f = len(want["boundary-faces-000.dat"].split())
+        assert nb_f == 2 * (2 * nx * ny + 2 * nx * nz) + 2 * ny * nz
+
+
+@pytest.mark.gpu
+def test_driver_output_files_match_oracle(oracle, tmp_path):
+    """minipic_main --output 1 writes main.cpp:125-136's output.<t> files (one
+    line per sampled electron: x, v at default stream precision) before every
+    step, and --dump_mesh 1 the mesh files: against the oracle's particles of
+    fill index 0, nppe, 2 nppe, ... at the same times."""
+    say("minipic_main: output files")
+    nppe, dt, steps = 4, 0.1, 3
+    r = subprocess.run([DRIVER, "--nx", "2", "--nparts", str(nppe), "--steps", str(steps), "--dt", str(dt),
+                        "--output", "1", "--dump_mesh", "1", "--quiet", "1", "--capacity-factor", "2"],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    O = oracle
+    om = O.Mesh(2, 2, 2, 5, lo=(-1, -1, -1), hi=(1, 1, 1))
+    p = O.Particles(om.ne * nppe * 2)
+    p.fill(om, nppe, 0, DENSITY)  # main.cpp:108-109
+    p.fill(om, nppe, 1, DENSITY)
+    S = O.charge_scale(om, p, None)
+    phi = np.zeros(om.nn)
+    t = 0.0
+    for s in range(steps):
+        want = "".join("%s %s %s %s %s %s\n" % tuple("%g" % a for a in (*p.x[:, i], *p.v[:, i]))
+                       for i in range(0, p.n, nppe) if p.type[i] == 0)
+        got = (tmp_path / ("output.%f" % t)).read_text()
+        assert got == want, s
+        O.step(om, p, phi, dt, bg=None, S=S)
+        t += dt
+    assert (tmp_path / "nodes-000.dat").exists() and (tmp_path / "boundary-edges-000.dat").exists()
