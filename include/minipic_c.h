@@ -227,6 +227,16 @@ int mpic_weight_charge(mpic_ctx* ctx);                                 /* PIC::w
  * its early exits).  In mpic_time_step a BiCGStab that does not converge does
  * not stop the step (PIC.cpp:251 ignores it); cg_relres reports the residual. */
 int mpic_solve(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres);
+/* No counterpart in the reference: mpic_solve's FP64 CG for callers that
+ * cannot allocate device memory during a call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The solve's working
+ * memory (solver state, per-iteration history, reduction partials) is a
+ * file-scope device array of the library, one per GPU (about 400 KB, held
+ * until the process ends); the call allocates nothing.  Same results as
+ * mpic_solve bit for bit.  Limits: max_iter <= 2047 and at most 4,194,304
+ * nodes (MPIC_ERR_ARG beyond); FP64 form with the CG solver (MPIC_ERR_STATE
+ * otherwise).  Calls that overlap on one GPU race on the array. */
+int mpic_solve_noalloc(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres);
 int mpic_weight_efield(mpic_ctx* ctx);                                 /* PIC::weight_Efield, PIC.cpp:162-189 */
 int mpic_accel(mpic_ctx* ctx, double dt, double scale);                /* ParticleMove AccelParticlesTag, particle_move.hpp:364-392 */
 int mpic_move(mpic_ctx* ctx, double dt, mpic_move_stats* stats);       /* MoveParticle::execute, particle_move.hpp:72-124 */

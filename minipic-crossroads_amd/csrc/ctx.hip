@@ -303,15 +303,23 @@ int do_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
       c->persist_off = true;  // the per-iteration kernels from here on (x is unchanged)
     }
   }
-  HIPCHK(c, launch_cg_init(c->m, c->cg, tol, max_iter, c->stream));
-  // Launch iterations in batches sized from the previous solve; kernels after
-  // convergence return at once.
+  return cg_run(c, c->cg, tol, max_iter, iters, relres);
+}
+
+// The per-iteration CG (CG_solve.hpp:54-114) on buffers b: iterations are
+// launched in batches sized from the previous solve (kernels after
+// convergence return at once), the state is read after each batch.  Allocates
+// nothing.
+int cg_run(mpic_ctx* c, CGBuffers& b, double tol, int max_iter, int* iters, double* relres) {
+  b.x = c->phi;
+  b.b = c->rhs;
+  HIPCHK(c, launch_cg_init(c->m, b, tol, max_iter, c->stream));
   int it = 0;
   int batch = std::max(8, std::min(max_iter + 1, c->last_cg_iters + 1));
   while (true) {
     const int upto = std::min(max_iter + 1, it + batch);
-    for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, c->cg, it, max_iter, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->cg_h, c->cg.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+    for (; it < upto; ++it) HIPCHK(c, launch_cg_iter(c->m, b, it, max_iter, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cg_h, b.st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, wait_stream(c->stream));
     if (c->cg_h->done || it > max_iter) break;
     batch = 8;
@@ -414,6 +422,8 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
 
   mpic_ctx* c = new mpic_ctx();
   c->device = dev;
+  // the address of the allocation-free solve's scratch on this device (once per device)
+  c->noalloc_scratch = noalloc_prepare(dev);
   if (cfg)
     c->cfg = *cfg;
   else

@@ -56,6 +56,7 @@ struct mpic_ctx {
   double* tr = nullptr;  // transient [cap]: time_remaining of the device view
   // device ParticleList view: counters [4] (particles, deleted, migrate
   // entries, dropped appends) and the migrate list
+  void* noalloc_scratch = nullptr;  // this device's static scratch of mpic_solve_noalloc (null: unavailable)
   unsigned long long* view_counters = nullptr;
   int64_t* migrate_index = nullptr;
   int32_t* migrate_proc = nullptr;
@@ -163,6 +164,11 @@ int finish_partials(mpic_ctx* c, int parts, PushPartial* out);
 int materialize(mpic_ctx* c);
 int rank_now(mpic_ctx* c, bool by_bucket, bool counts_ready, int64_t n_live);
 int rank_enqueue(mpic_ctx* c, bool by_bucket, bool counts_ready);
+// The per-iteration FP64 CG on buffers b (x = phi, b = rhs of c); allocates nothing.
+int cg_run(mpic_ctx* c, CGBuffers& b, double tol, int max_iter, int* iters, double* relres);
+// Device address of the allocation-free solve's static scratch on `device`
+// (queried once per device, with it current); null if the query failed.
+void* noalloc_prepare(int device);
 // The device ParticleList view of the FP64 set P[cur] (counters not reset).
 minipic::ParticleListView particle_view(mpic_ctx* c);
 void rank_finish(mpic_ctx* c, bool by_bucket);

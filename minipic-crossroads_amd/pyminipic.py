@@ -31,7 +31,7 @@ EXPORTS = [
     "mpic_batch_create", "mpic_batch_destroy", "mpic_batch_fill", "mpic_batch_tune", "mpic_batch_step",
     "mpic_batch_graph", "mpic_batch_problem_stats", "mpic_batch_get_phi", "mpic_batch_get_rhs",
     "mpic_batch_download", "mpic_batch_check_ownership", "mpic_batch_last_error",
-    "mpic_particles_view", "mpic_particles_view_commit", "mpic_stream",
+    "mpic_particles_view", "mpic_particles_view_commit", "mpic_stream", "mpic_solve_noalloc",
 ]
 
 
@@ -147,6 +147,7 @@ def lib():
             "mpic_particles_view": (i32, [P, P]),
             "mpic_particles_view_commit": (i32, [P, P, P]),
             "mpic_stream": (i32, [P, P]),
+            "mpic_solve_noalloc": (i32, [P, d, i32, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
