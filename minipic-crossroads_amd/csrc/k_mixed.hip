@@ -354,7 +354,14 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
         enew = alive ? e : -1;
         if (!alive) atomicAdd(&rare.killed, 1);
         rc.elem = enew;
-        store_rec(a.Q.r, p, rc);
+        if (gather) {
+          store_rec(a.Q.r, p, rc);
+        } else {
+          // in place: the (weight, id) word does not change
+          float4* q = reinterpret_cast<float4*>(a.Q.r + p);
+          q[0] = float4{rc.x[0], rc.x[1], rc.x[2], __int_as_float(rc.elem)};
+          q[1] = float4{rc.v[0], rc.v[1], rc.v[2], __int_as_float((int)rc.type)};
+        }
         a.Q.elem[p] = enew;
       } else if (gather) {
         store_rec(a.Q.r, p, rc);
