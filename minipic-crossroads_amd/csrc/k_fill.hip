@@ -12,12 +12,50 @@
 //   x = refToPhysical; velocity by set_vel (particle_fill.hpp:101-124);
 //   weight = density*(detJ/6)/nppe.
 // Extensions: seed += seed_offset * 0x9E3779B97F4A7C15 (0 = reference seeds);
-// weight *= 1 + pert_amp*sin(pert_k*(x0 - lo0)).
+// weight *= 1 + pert_amp*sin(pert_k*(x0 - lo0)).  Both sines are fill_sin.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
 
 namespace mpic {
 namespace {
+
+// The fill's sine (the beam perturbation of set_vel, particle_fill.hpp:106-111,
+// and the weight perturbation): a fixed sequence of IEEE operations, so the
+// fill gives the same bits wherever it runs.  Cody and Waite's reduction
+// x = k pi/2 + r, |r| <= pi/4, with pi/2 split into a 33-bit head and a tail
+// (the split of fdlibm's __ieee754_rem_pio2), then the Taylor polynomials of
+// sin r (to r^21) and cos r (to r^20) in Horner form in z = r^2; within 1 ulp
+// of a correctly rounded sine for |x| below a few thousand.  The file is built
+// with -ffp-contract=off, so no product is fused into an addition.
+__device__ __forceinline__ double fill_sin(double x) {
+  const double k = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
+  const double r = (x - k * 0x1.921fb54400000p+0) - k * 0x1.0b4611a626331p-34;
+  const double z = r * r;
+  double s = 1.9572941063391263e-20;
+  s = s * z + -8.22063524662433e-18;
+  s = s * z + 2.8114572543455206e-15;
+  s = s * z + -7.647163731819816e-13;
+  s = s * z + 1.6059043836821613e-10;
+  s = s * z + -2.505210838544172e-08;
+  s = s * z + 2.7557319223985893e-06;
+  s = s * z + -0.0001984126984126984;
+  s = s * z + 0.008333333333333333;
+  s = s * z + -0.16666666666666666;
+  const double sn = r + (r * z) * s;
+  double c = 4.110317623312165e-19;
+  c = c * z + -1.5619206968586225e-16;
+  c = c * z + 4.779477332387385e-14;
+  c = c * z + -1.1470745597729725e-11;
+  c = c * z + 2.08767569878681e-09;
+  c = c * z + -2.755731922398589e-07;
+  c = c * z + 2.48015873015873e-05;
+  c = c * z + -0.001388888888888889;
+  c = c * z + 0.041666666666666664;
+  c = c * z + -0.5;
+  const double cs = 1.0 + z * c;
+  const int q = (int)(long long)k & 3;
+  return q == 0 ? sn : q == 1 ? cs : q == 2 ? -sn : -cs;
+}
 
 // dRandom, random.hpp:43-87: s = A s + C mod 2^64, drand = (double)s * 2^-64.
 struct Lcg {
@@ -65,7 +103,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
       vv[1] = a.vel[1];
       vv[2] = a.vel[2];
     } else if (temp < 0) {
-      vv[0] = a.vel[0] * (1. + temp * sin(1. * 3.14159265358979323846 * ph[0]));
+      vv[0] = a.vel[0] * (1. + temp * fill_sin(1. * 3.14159265358979323846 * ph[0]));
       vv[1] = vv[2] = 0.0;
     } else {
       while (true) {
@@ -86,7 +124,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
     }
     const double vol = a.m.detj[e] / 6.;
     double wp = a.density * vol / a.nppe;
-    if (a.pert_amp != 0.0) wp = wp * (1.0 + a.pert_amp * sin(a.pert_k * (ph[0] - a.lo0)));
+    if (a.pert_amp != 0.0) wp = wp * (1.0 + a.pert_amp * fill_sin(a.pert_k * (ph[0] - a.lo0)));
     minipic::Particle& o = team_p[threadIdx.x];
     for (int d = 0; d < 3; ++d) {
       o.x[d] = ph[d];

@@ -56,6 +56,40 @@ struct LCG {
   double drand() { const double d = 1.0 / 18446744073709551616.0; return (double)next() * d; }
 };
 
+// The sine of the fill (the beam perturbation of set_vel, particle_fill.hpp:106-111,
+// and the configurations' weight perturbation).  The reference calls the
+// platform's sin, whose last bit differs between the device and the C library;
+// the build specifies this one: x = k pi/2 + r by Cody and Waite's reduction
+// (pi/2 = fdlibm's 33-bit head 0x1.921fb544p+0 plus its tail), then Taylor
+// polynomials of sin r (terms to r^21) and cos r (to r^20), Horner in z = r^2,
+// coefficients (-1)^k/(2k+1)! and (-1)^k/(2k)! rounded to double.  Pinned
+// against the C library's sin (within 1 ulp) in tests/test_oracle.py.
+double fill_sin(double x) {
+  const double k = std::rint(x * 0x1.45f306dc9c883p-1);
+  const double r = (x - k * 0x1.921fb54400000p+0) - k * 0x1.0b4611a626331p-34;
+  const double z = r * r;
+  static const double S[10] = {1.9572941063391263e-20, -8.22063524662433e-18, 2.8114572543455206e-15,
+                               -7.647163731819816e-13, 1.6059043836821613e-10, -2.505210838544172e-08,
+                               2.7557319223985893e-06, -0.0001984126984126984, 0.008333333333333333,
+                               -0.16666666666666666};
+  static const double C[10] = {4.110317623312165e-19, -1.5619206968586225e-16, 4.779477332387385e-14,
+                               -1.1470745597729725e-11, 2.08767569878681e-09, -2.755731922398589e-07,
+                               2.48015873015873e-05, -0.001388888888888889, 0.041666666666666664, -0.5};
+  double s = S[0], c = C[0];
+  for (int i = 1; i < 10; ++i) {
+    s = s * z + S[i];
+    c = c * z + C[i];
+  }
+  const double sn = r + (r * z) * s;
+  const double cs = 1.0 + z * c;
+  switch ((long long)k & 3) {
+    case 0: return sn;
+    case 1: return cs;
+    case 2: return -sn;
+    default: return -cs;
+  }
+}
+
 // 4-point degree-2 cubature on the reference tet (inferred Intrepid default,
 // used through ES.cpp:55,70): a = (5 - sqrt 5)/20, b = 1 - 3a, weight 1/24.
 const double CUB_A = 0.1381966011250105;
@@ -591,6 +625,9 @@ inline bool walk(const Mesh& m, int32_t& e, double x[3], double xr[3], double v[
 extern "C" {
 
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+void orc_fill_sin(int64_t n, const double* x, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = fill_sin(x[i]);
+}
 int orc_get_threads() { return omp_get_max_threads(); }
 
 void* orc_mesh_create(int nx, int ny, int nz, int split, const double* lo, const double* hi, double jitter,
@@ -718,7 +755,7 @@ int64_t orc_fill(void* h, int nppe, int type, double density, const double* vel,
         vv[1] = vel[1];
         vv[2] = vel[2];
       } else if (temp < 0) {
-        vv[0] = vel[0] * (1. + temp * std::sin(1. * M_PI * ph[0]));
+        vv[0] = vel[0] * (1. + temp * fill_sin(1. * M_PI * ph[0]));
         vv[1] = vv[2] = 0.0;
       } else {
         while (true) {
@@ -738,7 +775,7 @@ int64_t orc_fill(void* h, int nppe, int type, double density, const double* vel,
         }
       }
       double wp = wt;
-      if (pert_amp != 0.0) wp = wt * (1.0 + pert_amp * std::sin(kx * (ph[0] - m.lo[0])));
+      if (pert_amp != 0.0) wp = wt * (1.0 + pert_amp * fill_sin(kx * (ph[0] - m.lo[0])));
       for (int d = 0; d < 3; ++d) {
         x[d * stride + p] = ph[d];
         xr[d * stride + p] = ref[d];

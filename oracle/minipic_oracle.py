@@ -71,6 +71,7 @@ def lib():
         L.orc_step.restype = i32
         L.orc_step.argtypes = [P, i64, i64, P, P, P, P, P, P, P, P, P, i32, d, d, i32, i32, P, P, P]
         L.orc_set_threads.argtypes = [i32]
+        L.orc_fill_sin.argtypes = [i64, P, P]
         L.orc_get_threads.restype = i32
         _lib = L
     return _lib
@@ -78,6 +79,14 @@ def lib():
 
 def ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def fill_sin(x):
+    """The fill's sine (fill_sin of the oracle), elementwise over a float64 array."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().orc_fill_sin(x.size, ptr(x), ptr(out))
+    return out
 
 
 class Mesh:

@@ -10,7 +10,9 @@
                  reference's two-stream fills (main.cpp:98-106: beams of
                  type 0 at vel (+-0.5, 0, 0), temp -0.05, then type 2 at rest),
                  whose velocities come from set_vel's temp < 0 branch
-                 (particle_fill.hpp:106-111) with math.sin.
+                 (particle_fill.hpp:106-111) with the fill's sine (fill_sin
+                 below: the build specifies the sine's last bit, which the
+                 platform's sin leaves open).
   c1_solve.npz   the C1 problem's electron charge vector (oracle deposit) and the
                  Dirichlet Poisson solution from scipy.sparse.linalg.spsolve.
 
@@ -75,6 +77,25 @@ def ref_brick_elements(n):
     return els
 
 
+def fill_sin(x):
+    """The fill's sine, as specified for the build: Cody and Waite's reduction
+    x = k pi/2 + r with fdlibm's two-part pi/2 (head 0x1.921fb544p+0), then the
+    Taylor polynomials of sin r (to r^21) and cos r (to r^20) in Horner form in
+    z = r^2, each operation an IEEE double operation (Python never fuses)."""
+    k = float(round(x * float.fromhex("0x1.45f306dc9c883p-1")))
+    r = (x - k * float.fromhex("0x1.921fb54400000p+0")) - k * float.fromhex("0x1.0b4611a626331p-34")
+    z = r * r
+    s = (-1) ** 10 / math.factorial(21)
+    for j in range(9, 0, -1):
+        s = s * z + (-1) ** j / math.factorial(2 * j + 1)
+    c = (-1) ** 10 / math.factorial(20)
+    for j in range(9, 0, -1):
+        c = c * z + (-1) ** j / math.factorial(2 * j)
+    sn = r + (r * z) * s
+    cs = 1.0 + z * c
+    return (sn, cs, -sn, -cs)[int(k) & 3]
+
+
 def det3(J):
     return (J[0][0] * J[1][1] * J[2][2] + J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1]
             - J[0][2] * J[1][1] * J[2][0] - J[0][0] * J[1][2] * J[2][1] - J[0][1] * J[1][0] * J[2][2])
@@ -105,7 +126,7 @@ def fill_particle(els, elem, ipart, ptype, nppe, density, vel, temp, mass):
         v = list(vel)
     elif temp < 0:
         # beam with a velocity perturbation; no random draws
-        v = [vel[0] * (1. + temp * math.sin(1. * math.pi * ph[0])), 0.0, 0.0]
+        v = [vel[0] * (1. + temp * fill_sin(1. * math.pi * ph[0])), 0.0, 0.0]
     else:
         while True:
             r = dr()

@@ -186,6 +186,27 @@ def oracle_particles(oracle, g, idx=None):
     return P
 
 
+def test_full_size_c2_fill(mp, oracle):
+    """The device fill of the C2 workload (Kuhn 64^3, 64 electrons/tet, ~100M,
+    with the density perturbation, whose device sin and the rejection test's
+    device exp are the fill's only transcendental calls) against the oracle's
+    fill, every particle, bit for bit."""
+    n = 64
+    say("full size C2 fill: device")
+    pm = mp.Mesh(n, n, n, 6, background=DENSITY)
+    ctx = mp.Context(pm.desc, mp.default_config())
+    ctx.reserve(pm.ne * 64)
+    ctx.fill(64, 0, DENSITY, temp=0.02, seed_offset=1, pert_amp=0.05)
+    g = ctx.download()
+    ctx.close()
+    say("full size C2 fill: oracle")
+    om = oracle.Mesh(n, n, n, 6)
+    p = oracle.Particles(om.ne * 64).fill(om, 64, 0, DENSITY, temp=0.02, seed_offset=1, pert_amp=0.05)
+    assert p.n == pm.ne * 64
+    say("full size C2 fill: compare")
+    assert_particles_equal(g, p, "C2 fill")
+
+
 def test_full_size_c2_properties(mp, oracle):
     """Kuhn 64^3 (1.57M tets), 64 electrons/tet (~100M): device fill, two fused
     steps; per-cell counts, node charge and potential bit-identical to the

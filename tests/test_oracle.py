@@ -8,7 +8,12 @@ Each oracle function and the check that pins it:
                                  two-stream velocity branch (temp < 0) -> the
                                  same restatement's "two_stream" fills, plus
                                  v = (vel (1 + temp sin(pi x)), 0, 0) on every
-                                 particle with math.sin
+                                 particle with the fill's sine
+  fill's sine (fill_sin)      -> within 1 ulp (plus 1e-24 absolute, the
+                                 reduction's error next to multiples of pi) of
+                                 the C library's sin; exact zero and oddness;
+                                 bit-equal to the pure-Python restatement in
+                                 golden/make_golden.py
   mesh generators, geometry   -> volumes, det, J*Jinv = I, face convention,
                                  neighbour symmetry, boundary counts (mesh.cpp:641-647)
   node mappings 1-3           -> closed forms: 1 = rotation about z by
@@ -45,6 +50,14 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
+
+
+def golden_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLD, "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
 DENSITY = 2.0 / 3.0 * math.pi * math.pi  # main.cpp:68
 
 
@@ -106,13 +119,35 @@ def test_fill_two_stream_known_answers(oracle):
     for k, f in enumerate(kat[:2]):
         sl = slice(offsets[k], offsets[k] + nb)
         assert not p.v[1:, sl].any()
-        want = [f["vel"][0] * (1.0 + f["temp"] * math.sin(math.pi * x)) for x in p.x[0, sl]]
+        fs = golden_module().fill_sin
+        want = [f["vel"][0] * (1.0 + f["temp"] * fs(math.pi * x)) for x in p.x[0, sl]]
         assert np.array_equal(p.v[0, sl], np.array(want))
+        libm = np.array([f["vel"][0] * (1.0 + f["temp"] * math.sin(math.pi * x)) for x in p.x[0, sl]])
+        assert np.all(np.abs(p.v[0, sl] - libm) <= np.spacing(np.abs(libm)))
         assert abs(p.v[0, sl]).min() > 0.5 * (1 - 0.05) - 1e-15 and abs(p.v[0, sl]).max() < 0.5 * (1 + 0.05) + 1e-15
     assert np.array_equal(p.x[:, :nb], p.x[:, nb:2 * nb])
     assert np.array_equal(p.v[0, :nb], -p.v[0, nb:2 * nb])
     # ions at rest
     assert not p.v[:, 2 * nb:p.n].any()
+
+
+def test_fill_sin(oracle):
+    """The fill's sine against the C library's sin over the fill's arguments
+    (2 pi (x - lo)/Lx in [0, 2 pi], pi x in [-pi, pi]) and beyond, and against
+    its pure-Python restatement bit for bit."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-8.0, 8.0, 400000), np.linspace(-4 * np.pi, 4 * np.pi, 8193),
+                        np.pi / 4 * np.arange(-16, 17), rng.uniform(-1e-6, 1e-6, 1000)])
+    s = oracle.fill_sin(x)
+    ref = np.sin(x)
+    assert np.all(np.abs(s - ref) <= np.spacing(np.abs(ref)) + 1e-24)
+    assert np.array_equal(oracle.fill_sin(-x), -s)
+    assert oracle.fill_sin(np.array([0.0]))[0] == 0.0
+    # a slip in a coefficient or the reduction shows up at this level
+    assert np.max(np.abs(s - ref)) < 3e-16
+    fs = golden_module().fill_sin
+    sub = x[::97]
+    assert np.array_equal(oracle.fill_sin(sub), np.array([fs(float(v)) for v in sub]))
 
 
 def test_fill_statistics(oracle):
