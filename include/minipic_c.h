@@ -69,6 +69,11 @@ typedef struct {
   const double* background;     /* [num_nodes] constant rhs term (ES::set_analytic_RHS, ES.cpp:235-257) or NULL */
   const int32_t* node_pair;     /* [num_nodes] or NULL: for a periodic image node, the node whose row carries its
                                    charge and stiffness (Mesh::node_pairs, mesh.cpp:178-189), else -1 */
+  int32_t lattice[3];           /* cells (nx, ny, nz) when the nodes are the brick lattice numbered
+                                   (i*(ny+1) + j)*(nz+1) + k (mesh.cpp:52-54) with the Dirichlet nodes on its walls
+                                   (mpic_mesh_describe sets it for non-periodic bricks); all 0 otherwise.  The
+                                   precision form's multigrid solve uses it (checked against num_nodes and the
+                                   boundary flags; ignored when they disagree). */
 } mpic_mesh_desc;
 
 typedef struct {
@@ -86,6 +91,10 @@ typedef struct {
   int persistent_solve;                  /* 1: a solve is one cooperative launch when its grid fits on the GPU
                                             (same results, but slower on MI355X: its grid-wide waits cost more
                                             than a kernel boundary); 0 (default): one launch per iteration */
+  int multigrid;                         /* precision form: 1 (default) solves with conjugate gradients in FP64
+                                            preconditioned by an FP32 multigrid V-cycle when the mesh descriptor names
+                                            a lattice that coarsens (mpic_mesh_desc.lattice); 0, or a mesh without
+                                            one: the FP32 refinement CG.  Same stopping rule |r| < cg_tol |b|. */
 } mpic_config;
 
 /* Host particle SoA (planes as described above).  Borrowed for the call. */
@@ -227,6 +236,13 @@ int mpic_weight_charge(mpic_ctx* ctx);                                 /* PIC::w
  * its early exits).  In mpic_time_step a BiCGStab that does not converge does
  * not stop the step (PIC.cpp:251 ignores it); cg_relres reports the residual. */
 int mpic_solve(mpic_ctx* ctx, double tol, int max_iter, int* iters, double* relres);
+/* Which solve ES::solve runs in the precision form (no counterpart in the
+ * reference, whose solver is fixed): *levels = the multigrid levels of its
+ * preconditioned CG (fine mesh included), or 0 when it runs the refinement CG
+ * (FP64 form, multigrid switched off, or a mesh whose descriptor names no
+ * lattice that coarsens; mpic_last_error then gives the reason).  Prepares the
+ * precision form's tables when needed. */
+int mpic_multigrid_levels(mpic_ctx* ctx, int* levels);
 /* No counterpart in the reference: mpic_solve's FP64 CG for callers that
  * cannot allocate device memory during a call (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The solve's working

@@ -396,6 +396,7 @@ void mpic_config_default(mpic_config* cfg) {
   cfg->sort_period = 1;
   cfg->sort_bucket = 1;
   cfg->persistent_solve = 0;
+  cfg->multigrid = 1;
   cfg->num_species = 3;
   for (int t = 0; t < MPIC_MAX_SPECIES; ++t) cfg->species_m[t] = 1.0;
   cfg->species_q[0] = -1.0;  // e-  (particle_type.hpp:69-71)
@@ -481,6 +482,16 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     return bail(fail(c, MPIC_ERR_ARG,
                      "mpic_create: a periodic mesh has a non-symmetric stiffness matrix; use MPIC_SOLVER_BICGSTAB "
                      "(the reference's solver) in the FP64 form"));
+  // A lattice mesh keeps its CSR on the host for the precision form's multigrid
+  // hierarchy (built when that form is first prepared).
+  if (c->cfg.multigrid && !c->periodic && d->lattice[0] > 0 && d->lattice[1] > 0 && d->lattice[2] > 0 &&
+      (int64_t)(d->lattice[0] + 1) * (d->lattice[1] + 1) * (d->lattice[2] + 1) == nn) {
+    for (int a = 0; a < 3; ++a) c->lattice[a] = d->lattice[a];
+    c->mg_rowptr.assign(d->row_ptr, d->row_ptr + nn + 1);
+    c->mg_col.assign(d->col_idx, d->col_idx + d->nnz);
+    c->mg_val.assign(d->values, d->values + d->nnz);
+    c->mg_bnd.assign(d->boundary, d->boundary + nn);
+  }
   std::vector<int32_t> adj_ptr(nn + 1, 0), adj((size_t)ne * 4);
   for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj_ptr[row_of[s] + 1]++;
   for (int32_t i = 0; i < nn; ++i) adj_ptr[i + 1] += adj_ptr[i];
@@ -657,6 +668,8 @@ void mpic_destroy(mpic_ctx* c) {
   if (c->ir.rho_hist) (void)hipFree(c->ir.rho_hist);
   if (c->ir.alpha_hist) (void)hipFree(c->ir.alpha_hist);
   if (c->start_h) (void)hipHostFree(c->start_h);
+  if (c->mg_h) (void)hipHostFree(c->mg_h);
+  if (c->mg.rz_hist) (void)hipFree(c->mg.rz_hist);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1165,6 +1178,22 @@ int mpic_particles_view_commit(mpic_ctx* c, int64_t* num_particles, int64_t* num
   if (cnt[3] > 0)
     return fail(c, MPIC_ERR_CAPACITY, "ParticleList::push_back: " + std::to_string(cnt[3]) +
                                           " particle(s) beyond the capacity were not added");
+  return MPIC_OK;
+}
+
+int mpic_multigrid_levels(mpic_ctx* c, int* levels) {
+  if (!c || !levels) return MPIC_ERR_ARG;
+  *levels = 0;
+  if (c->cfg.form != MPIC_FORM_MIXED && !c->state32) {
+    c->err = "the FP64 form solves with the reference's solver";
+    return MPIC_OK;
+  }
+  int rc;
+  if ((rc = mixed_prepare(c))) return rc;
+  if (c->mg_ok)
+    *levels = c->mg.nlev + 1;
+  else
+    c->err = c->mg_why;
   return MPIC_OK;
 }
 

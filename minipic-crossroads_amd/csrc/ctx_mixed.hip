@@ -57,6 +57,126 @@ void record_times(mpic_ctx* c) {
   }
 }
 
+// The multigrid hierarchy of the solve on the device, once per context; the
+// host copy of the matrix is released afterwards.  A mesh without a lattice
+// (or one that does not coarsen) keeps the refinement CG.
+int mg_prepare(mpic_ctx* c) {
+  c->mg_tried = true;
+  if (c->mg_rowptr.empty()) {
+    c->mg_why = c->cfg.multigrid ? "the mesh descriptor names no lattice" : "multigrid switched off";
+    return MPIC_OK;
+  }
+  MGHost h;
+  std::string why;
+  const bool ok = mg_build(c->lattice[0], c->lattice[1], c->lattice[2], c->m.nn, c->mg_bnd.data(),
+                           c->mg_rowptr.data(), c->mg_col.data(), c->mg_val.data(), h, why);
+  std::vector<int32_t>().swap(c->mg_rowptr);
+  std::vector<int32_t>().swap(c->mg_col);
+  std::vector<double>().swap(c->mg_val);
+  std::vector<uint8_t>().swap(c->mg_bnd);
+  if (!ok || h.levels.size() > (size_t)MG_MAX_LEVELS) {
+    c->mg_why = ok ? "too many levels" : why;
+    return MPIC_OK;
+  }
+  MGBuffers& g = c->mg;
+  g.nx = h.nx;
+  g.ny = h.ny;
+  g.nz = h.nz;
+  g.nlev = (int)h.levels.size();
+  g.cb = g.nlev;
+  for (int l = 1; l <= g.nlev; ++l)
+    if (h.levels[l - 1].n <= MG_CB_MAX) {
+      g.cb = l;
+      break;
+    }
+  g.nchunks = c->cg.nchunks;
+  const int32_t nn = c->m.nn;
+  int rc;
+  for (int l = 1; l <= g.nlev; ++l) {
+    const MGHostLevel& H = h.levels[l - 1];
+    MGLevelDev& L = g.host_lev[l];
+    int32_t* col;
+    float *val, *dinv;
+    L.cx = H.cx;
+    L.cy = H.cy;
+    L.cz = H.cz;
+    L.n = H.n;
+    if ((rc = dupload(c, &col, H.col.data(), H.col.size())) || (rc = dupload(c, &val, H.val.data(), H.val.size())) ||
+        (rc = dupload(c, &dinv, H.dinv.data(), H.dinv.size())) || (rc = dalloc(c, &L.b, (size_t)H.n)) ||
+        (rc = dalloc(c, &L.x, (size_t)H.n)) || (rc = dalloc(c, &L.t, (size_t)H.n)))
+      return rc;
+    L.col = col;
+    L.val = val;
+    L.dinv = dinv;
+  }
+  MGLevelDev* lev;
+  float *dinv0, *cinv;
+  if ((rc = dupload(c, &lev, g.host_lev, (size_t)g.nlev + 1)) ||
+      (rc = dupload(c, &dinv0, h.dinv0.data(), h.dinv0.size())) ||
+      (rc = dupload(c, &cinv, h.coarse_inv.data(), h.coarse_inv.size())) || (rc = dalloc(c, &g.b0, (size_t)nn)) ||
+      (rc = dalloc(c, &g.x0, (size_t)nn)) || (rc = dalloc(c, &g.t0, (size_t)nn)) || (rc = dalloc(c, &g.r, (size_t)nn)) ||
+      (rc = dalloc(c, &g.q, (size_t)nn)) || (rc = dalloc(c, &g.p[0], (size_t)nn)) ||
+      (rc = dalloc(c, &g.p[1], (size_t)nn)) || (rc = dalloc(c, &g.part_bb, (size_t)g.nchunks)) ||
+      (rc = dalloc(c, &g.part_rr, (size_t)g.nchunks)) || (rc = dalloc(c, &g.part_rz, (size_t)g.nchunks)) ||
+      (rc = dalloc(c, &g.part_pq, (size_t)g.nchunks)) || (rc = dalloc(c, &g.st, 1)))
+    return rc;
+  g.lev = lev;
+  g.dinv0 = dinv0;
+  g.cinv = cinv;
+  if (!c->mg_h && hipHostMalloc((void**)&c->mg_h, sizeof(MGState)) != hipSuccess)
+    return fail(c, MPIC_ERR_HIP, "pinned host allocation failed");
+  c->mg_ok = true;
+  return MPIC_OK;
+}
+
+// Multigrid-preconditioned CG (k_mg.hip): the iterations the previous solve
+// took are queued at once, then the state is read; after that two at a time.
+// When the recursive residual meets the tolerance, the true residual b - A x is
+// formed in FP64 and tested; if it misses, the iteration restarts from it.
+int mg_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
+  MGBuffers& g = c->mg;
+  if (max_iter + 2 > c->mg_hist_cap) {
+    if (g.rz_hist) (void)hipFree(g.rz_hist);
+    g.rz_hist = nullptr;
+    HIPCHK(c, hipMalloc((void**)&g.rz_hist, sizeof(double) * (size_t)(max_iter + 2)));
+    c->mg_hist_cap = max_iter + 2;
+  }
+  HIPCHK(c, launch_mg_start(c->m, c->m32, g, c->phi, c->rhs, tol, max_iter, c->stream));
+  int it = 0;  // the next iteration to queue
+  bool restart = false;
+  int rechecks = 0;
+  int batch = c->mg_last_iters > 0 ? c->mg_last_iters : 12;
+  for (;;) {
+    for (int k = 0; k < batch && it < max_iter; ++k, ++it) {
+      HIPCHK(c, launch_mg_iter(c->m, c->m32, g, c->phi, c->rhs, it, restart, c->stream));
+      restart = false;
+    }
+    batch = 2;
+    HIPCHK(c, hipMemcpyAsync(c->mg_h, g.st, sizeof(MGState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, wait_stream(c->stream));
+    const MGState s = *c->mg_h;
+    if (!s.done) {
+      // the V-cycle after iteration max_iter - 1 stops the solve
+      if (it >= max_iter) return fail(c, MPIC_ERR_STATE, "multigrid CG: no stop at the iteration limit");
+      continue;
+    }
+    if (!s.converged) break;  // iteration limit or breakdown
+    HIPCHK(c, launch_mg_recheck(c->m, c->m32, g, c->phi, c->rhs, s.iters, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->mg_h, g.st, sizeof(MGState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, wait_stream(c->stream));
+    if (c->mg_h->done || ++rechecks > MG_MAX_RECHECKS) break;
+    it = s.iters;  // continue from the true residual
+    restart = true;
+  }
+  const MGState& s = *c->mg_h;
+  c->mg_last_iters = s.iters;
+  c->last_cg_iters = s.iters;
+  if (iters) *iters = s.iters;
+  if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  if (!(s.done && s.converged)) return fail(c, MPIC_ERR_NOT_CONVERGED, "multigrid CG did not converge");
+  return MPIC_OK;
+}
+
 }  // namespace
 
 int mixed_prepare(mpic_ctx* c) {
@@ -103,6 +223,8 @@ int mixed_prepare(mpic_ctx* c) {
     c->m32.ell_val64 = val64p;
     c->mixed_ready = true;
   }
+  int rc;
+  if (!c->mg_tried && (rc = mg_prepare(c))) return rc;
   return alloc_particles32(c);
 }
 
@@ -170,6 +292,7 @@ int mixed_weight_charge(mpic_ctx* c) {
 int mixed_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) {
   int rc;
   if ((rc = mixed_prepare(c))) return rc;
+  if (c->mg_ok) return mg_solve(c, tol, max_iter, iters, relres);
   c->ir.x = c->phi;
   c->ir.b = c->rhs;
   if (max_iter + 1 > c->ir_hist_cap) {

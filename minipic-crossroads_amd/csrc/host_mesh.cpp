@@ -417,6 +417,9 @@ extern "C" int mpic_mesh_describe(const mpic_mesh* m, mpic_mesh_desc* d) {
   d->values = m->val.data();
   d->background = m->has_background ? m->background.data() : nullptr;
   d->node_pair = m->periodic ? m->node_pair.data() : nullptr;
+  d->lattice[0] = m->periodic ? 0 : m->nx;
+  d->lattice[1] = m->periodic ? 0 : m->ny;
+  d->lattice[2] = m->periodic ? 0 : m->nz;
   return MPIC_OK;
 }
 

@@ -87,6 +87,20 @@ struct mpic_ctx {
   SortBuffers sb{};
   int32_t* start_h = nullptr;  // pinned [1]
 
+  // precision form's multigrid solve: the host CSR kept from mpic_create for the
+  // hierarchy's build (done at the first preparation of the precision form)
+  int lattice[3] = {0, 0, 0};
+  std::vector<int32_t> mg_rowptr, mg_col;
+  std::vector<double> mg_val;
+  std::vector<uint8_t> mg_bnd;
+  bool mg_tried = false;  // the build ran (mg_ok: it succeeded)
+  bool mg_ok = false;
+  std::string mg_why;     // why multigrid does not apply
+  mpic::MGBuffers mg{};
+  mpic::MGState* mg_h = nullptr;  // pinned
+  int mg_hist_cap = 0;
+  int mg_last_iters = 0;
+
   // precision form
   bool mixed_ready = false;  // tables allocated
   bool state32 = false;      // the live particle state is P32[cur32]

@@ -10,6 +10,7 @@
 
 #include "minipic_c.h"
 #include "minipic/particle_list_view.hpp"
+#include "mg.hpp"
 
 namespace mpic {
 
@@ -394,6 +395,52 @@ hipError_t launch_ir_setup(IRBuffers& c, double tol, int max_iter, hipStream_t s
 // results as launch_ir_outer / launch_ir_inner / launch_ir_update.
 hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, double tol, int max_iter,
                              double in_tol_min, int pass_len, unsigned* epoch, int* err, hipStream_t s);
+
+// ---- multigrid-preconditioned CG of the precision form (k_mg.hip, mg.hpp) ----
+// One coarse level on the device: lattice cells, unknowns, the 27-entry ELL
+// ([27][n] column-major), the diagonal inverse and three work vectors: b (the
+// restricted residual, overwritten by the level's V-cycle result), x = omega
+// D^-1 b (the first smoothing step) and t (the residual after it, then the
+// prolonged correction).
+struct MGLevelDev {
+  int cx, cy, cz, n;
+  const int32_t* col;
+  const float* val;
+  const float* dinv;
+  float *b, *x, *t;
+};
+struct MGState {
+  int done, converged, iters, max_iter;
+  double tol2, bb, rr;
+};
+struct MGBuffers {
+  int nlev = 0;             // coarse levels 1..nlev; the last is solved by its dense inverse
+  int cb = 0;               // first level of the single-workgroup kernel (levels cb..nlev)
+  int nx = 0, ny = 0, nz = 0;
+  int nchunks = 0;          // level 0 row chunks of BLOCK rows (partials are indexed by chunk)
+  const MGLevelDev* lev = nullptr;            // device [nlev + 1] (entry 0 unused)
+  MGLevelDev host_lev[MG_MAX_LEVELS + 1] = {};  // the same on the host (grid sizes)
+  const float* dinv0 = nullptr;  // [nn]
+  const float* cinv = nullptr;   // [n_nlev][n_nlev]
+  float *b0 = nullptr, *x0 = nullptr, *t0 = nullptr;  // level 0 vectors (b0 also receives z)
+  double *r = nullptr, *q = nullptr;
+  double* p[2] = {nullptr, nullptr};  // search directions by iteration parity
+  double *part_bb = nullptr, *part_rr = nullptr, *part_rz = nullptr, *part_pq = nullptr;
+  double* rz_hist = nullptr;  // [max_iter + 1]
+  MGState* st = nullptr;
+};
+// Resets the state, r = b - A x (FP64), and the V-cycle of iteration 0 (its
+// first kernel tests |r| < tol |b| and the iteration limit).
+hipError_t launch_mg_start(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, double tol,
+                           int max_iter, hipStream_t s);
+// Iteration it: q = A p (p = z + beta p_old; beta = 0 when restart), the x and r
+// update, then the V-cycle of iteration it + 1 (which first tests convergence).
+hipError_t launch_mg_iter(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+                          bool restart, hipStream_t s);
+// After a stop: reopens the solve, recomputes r = b - A x in FP64 and runs the
+// V-cycle of iteration it, whose first kernel tests the true residual.
+hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+                             hipStream_t s);
 
 struct SortBuffers {
   int32_t* count;    // [ne+1]

@@ -32,6 +32,7 @@ EXPORTS = [
     "mpic_batch_graph", "mpic_batch_problem_stats", "mpic_batch_get_phi", "mpic_batch_get_rhs",
     "mpic_batch_download", "mpic_batch_check_ownership", "mpic_batch_last_error",
     "mpic_particles_view", "mpic_particles_view_commit", "mpic_stream", "mpic_solve_noalloc",
+    "mpic_multigrid_levels",
 ]
 
 
@@ -41,14 +42,14 @@ class MeshDesc(C.Structure):
                 ("normals", C.c_void_p), ("inv_jacobian", C.c_void_p), ("det_jacobian", C.c_void_p),
                 ("grad_basis", C.c_void_p), ("wbv", C.c_void_p), ("boundary", C.c_void_p),
                 ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("values", C.c_void_p),
-                ("background", C.c_void_p), ("node_pair", C.c_void_p)]
+                ("background", C.c_void_p), ("node_pair", C.c_void_p), ("lattice", C.c_int32 * 3)]
 
 
 class Config(C.Structure):
     _fields_ = [("form", C.c_int), ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("kill_on_wall", C.c_int),
                 ("sort_period", C.c_int), ("sort_bucket", C.c_int), ("num_species", C.c_int),
                 ("species_q", C.c_double * MAX_SPECIES), ("species_m", C.c_double * MAX_SPECIES),
-                ("solver", C.c_int), ("persistent_solve", C.c_int)]
+                ("solver", C.c_int), ("persistent_solve", C.c_int), ("multigrid", C.c_int)]
 
 
 class SoaHost(C.Structure):
@@ -148,6 +149,7 @@ def lib():
             "mpic_particles_view_commit": (i32, [P, P, P]),
             "mpic_stream": (i32, [P, P]),
             "mpic_solve_noalloc": (i32, [P, d, i32, P, P]),
+            "mpic_multigrid_levels": (i32, [P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -305,6 +307,12 @@ class Context:
         if check:
             self._chk(rc)
         return rc == OK, it.value, rr.value
+
+    def multigrid_levels(self):
+        """(levels, reason): the precision form's multigrid levels, or 0 and why not."""
+        lv = C.c_int()
+        self._chk(lib().mpic_multigrid_levels(self.h, C.byref(lv)))
+        return lv.value, ("" if lv.value else lib().mpic_last_error(self.h).decode())
 
     def weight_efield(self):
         self._chk(lib().mpic_weight_efield(self.h))

@@ -296,3 +296,20 @@ def test_driver_output_files_match_oracle(oracle, tmp_path):
         O.step(om, p, phi, dt, bg=None, S=S)
         t += dt
     assert (tmp_path / "nodes-000.dat").exists() and (tmp_path / "boundary-edges-000.dat").exists()
+
+
+MG_CHECK = os.path.join(ROOT, "tests", "cpp", "mg_check")
+
+
+def test_multigrid_hierarchy_cpu():
+    """The precision form's multigrid hierarchy on the CPU (tests/cpp/mg_check.cpp):
+    Galerkin coarse operators equal to a dense P^T A P, V-cycle preconditioned
+    CG iteration counts independent of the mesh size (Kuhn and 5-tet splits, a
+    node mapping, jitter), refusal of odd lattices and periodic meshes."""
+    if not os.path.exists(MG_CHECK):
+        pytest.skip("tests/cpp not built (make -C tests/cpp)")
+    out = subprocess.run([MG_CHECK], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "mg_check ok" in out.stdout
+    its = [int(l.split("multigrid ")[1].split()[0]) for l in out.stdout.splitlines() if l.startswith("pcg ")]
+    assert len(its) == 5 and max(its) <= 25
