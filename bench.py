@@ -52,6 +52,54 @@ def cg_bytes_per_iter(n, nnz, word):
     return spmv, spmv + 5 * word * n
 
 
+def mg_levels_sizes(cells):
+    """Unknowns of the coarse levels of the multigrid solve on a cubic brick of
+    `cells` cells per direction: halve while even and >= 4; the level with at
+    most 512 unknowns is the coarsest (dense inverse)."""
+    sizes = []
+    c = cells
+    while c % 2 == 0 and c >= 4:
+        c //= 2
+        sizes.append((c - 1) ** 3)
+        if sizes[-1] <= 512:
+            break
+    return sizes
+
+
+def mg_bytes_per_iter(cells, n, nnz):
+    """Algorithmic bytes of one multigrid-preconditioned CG iteration of the
+    precision form (k_mg.hip), split as (FP64 CG part, V-cycle part):
+      q = A p: FP64 values and int32 columns (12 nnz), z (4 n) and the old p
+        (8 n) read, p and q written (16 n); the update: x, p, r, q read (32 n),
+        x, r written (16 n), the V-cycle's b0 and x0 written (8 n);
+      level 0 of the V-cycle: two FP32 SpMVs (8 nnz each) with b0, x0, t0, dinv,
+        r and z read or written once (32 n), restriction and prolongation
+        (t0, x0 read, t0 written: 12 n);
+      coarse levels of n_l unknowns: two 27-entry FP32 SpMVs (2 x 216 n_l) and
+        their vectors (40 n_l); the coarsest level's dense inverse (4 n_c^2)."""
+    cg = 12 * nnz + 28 * n + 56 * n
+    sizes = mg_levels_sizes(cells)
+    vc = 16 * nnz + 44 * n
+    for nl in sizes[:-1]:
+        vc += 2 * 216 * nl + 40 * nl
+    if sizes:
+        vc += 4 * sizes[-1] ** 2 + 16 * sizes[-1]
+    return cg, vc
+
+
+def mg_roofline(sm, cells, n, nnz, levels):
+    cg, vc = mg_bytes_per_iter(cells, n, nnz)
+    per_iter = cg + vc
+    it, ms = sm["cg_iters_mean"], sm["cg_ms_per_step"]
+    achieved = per_iter * it / (ms * 1e-3) / 1e9 if ms > 0 and it > 0 else 0.0
+    return {"kernel": "multigrid-preconditioned CG: k_mg_ap/k_mg_update (FP64) + the FP32 V-cycle (%d levels)" % levels,
+            "bound": "hbm", "bytes_per_iter": per_iter, "cg_bytes_per_iter": cg, "vcycle_bytes_per_iter": vc,
+            "iters_per_step": it, "ms_per_step": ms, "us_per_iter": ms * 1e3 / it if it else None,
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "time_covers": "the whole solve of a step (the first V-cycle, the iterations and the final check of the "
+                           "true FP64 residual), host waits included"}
+
+
 def cg_roofline(sm, n, nnz, word, kernel):
     spmv, per_iter = cg_bytes_per_iter(n, nnz, word)
     it, ms = sm["cg_iters_mean"], sm["cg_ms_per_step"]
@@ -132,8 +180,9 @@ def run_gpu(args, form):
         log("[%s] step %d: cg %d iters %.2f ms, deposit %.2f, push %.2f, sort %.2f, KE %.9e" % (
             name, i, st["cg_iters"], st["ms_solve"], st["ms_deposit"], st["ms_push"], st["ms_sort"],
             st["kinetic_energy"]))
+    mg_levels = ctx.multigrid_levels()[0] if form == mp.FORM_MIXED else 0
     out = dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nnz,
-               owner_mismatch=ctx.check_ownership())
+               owner_mismatch=ctx.check_ownership(), mg_levels=mg_levels, cells=n)
     ctx.close()
     return out
 
@@ -353,8 +402,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64 (precision form: f32 push/gather/walk and inner CG, f64 refinement and point location, "
-                 "i64 deposit)",
+        "dtype": "f32+f64 (precision form: f32 push/gather/walk, f64 CG preconditioned by an f32 multigrid V-cycle, "
+                 "f64 point location, i64 deposit)",
         "data": "synthetic (seeded device fill)",
         "config": {
             "workload": "Kuhn %d^3 x6 tets (%d), %d electrons/tet (%d particles), vth 0.1, density perturbation "
@@ -376,8 +425,9 @@ def main():
                                    "unit": "GB/s", "frac": sd["push_gbs"] / HBM_PEAK_GBS,
                                    "bytes_per_particle": PUSH_BYTES_FP64,
                                    "traffic": measured_traffic("k_push_fused")}},
-        "cg_roofline": cg_roofline(sm, gm["nn"], gm["nnz"], 4, "k_ir_cg (FP32 inner) + k_ir_resid/k_ir_start/"
-                                                                "k_ir_update (FP64 refinement)"),
+        "cg_roofline": (mg_roofline(sm, gm["cells"], gm["nn"], gm["nnz"], gm["mg_levels"]) if gm["mg_levels"] else
+                        cg_roofline(sm, gm["nn"], gm["nnz"], 4, "k_ir_cg (FP32 inner) + k_ir_resid/k_ir_start/"
+                                                               "k_ir_update (FP64 refinement)")),
         "mixed_over_fp64": sm["value"] / sd["value"],
         "errors_vs_fp64": errors,
         "roofline": {"bound": "hbm", "kernel": "k_push32", "achieved": sm["push_gbs"], "peak": HBM_PEAK_GBS,

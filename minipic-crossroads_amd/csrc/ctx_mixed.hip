@@ -165,7 +165,9 @@ int mg_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
     HIPCHK(c, hipMemcpyAsync(c->mg_h, g.st, sizeof(MGState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, wait_stream(c->stream));
     if (c->mg_h->done || ++rechecks > MG_MAX_RECHECKS) break;
-    it = s.iters;  // continue from the true residual
+    // continue from the true residual
+    HIPCHK(c, launch_mg_vcycle(c->m, c->m32, g, c->phi, c->rhs, s.iters, c->stream));
+    it = s.iters;
     restart = true;
   }
   const MGState& s = *c->mg_h;

@@ -199,6 +199,24 @@ __global__ void k_mg_reopen(MGState* st) {
   st->converged = 0;
 }
 
+// The stop test of iteration it on the partials of k_mg_init, in one workgroup
+// (the check of the true residual after the recursive one met the tolerance).
+__global__ __launch_bounds__(BLOCK) void k_mg_test(MGArgs a) {
+  MGState* st = a.g.st;
+  const double rr = reduce(a.g.part_rr, a.g.nchunks);
+  const double bb = reduce(a.g.part_bb, a.g.nchunks);
+  if (threadIdx.x == 0) {
+    const bool conv = rr == 0.0 || rr < st->tol2 * bb;
+    st->rr = rr;
+    st->bb = bb;
+    st->iters = a.it;
+    if (conv || a.it >= st->max_iter) {
+      st->done = 1;
+      st->converged = conv ? 1 : 0;
+    }
+  }
+}
+
 // r = b - A x (FP64), b0 = (float) r, x0 = omega D^-1 b0; partials of b.b and r.r.
 __global__ __launch_bounds__(BLOCK) void k_mg_init(MGArgs a) {
   const int chunk = xcd_chunk(blockIdx.x, a.g.nchunks);
@@ -299,10 +317,23 @@ __global__ __launch_bounds__(CB_THREADS) void k_mg_coarse(MGArgs a) {
   }
   {
     const MGLevelDev& C = a.g.lev[nlev];
-    const int n = C.n;  // <= MG_DIRECT_MAX < CB_THREADS
+    const int n = C.n;  // <= MG_DIRECT_MAX <= CB_THREADS: one row of the dense inverse per thread
     float z = 0.0f;
-    if (tid < n)
-      for (int j = 0; j < n; ++j) z += a.g.cinv[tid * n + j] * C.b[j];
+    if (tid < n) {
+      // the inverse is stored column-major: (tid, j) at j * n + tid
+      const float* __restrict__ col = a.g.cinv + tid;
+      const float* __restrict__ b = C.b;
+      int j = 0;
+      float z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
+      for (; j + 4 <= n; j += 4) {
+        z += col[(size_t)j * n] * b[j];
+        z1 += col[(size_t)(j + 1) * n] * b[j + 1];
+        z2 += col[(size_t)(j + 2) * n] * b[j + 2];
+        z3 += col[(size_t)(j + 3) * n] * b[j + 3];
+      }
+      for (; j < n; ++j) z += col[(size_t)j * n] * b[j];
+      z = (z + z1) + (z2 + z3);
+    }
     __syncthreads();
     if (tid < n) C.b[tid] = z;
     __syncthreads();
@@ -480,7 +511,15 @@ hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, 
   hipLaunchKernelGGL(k_mg_reopen, dim3(1), dim3(1), 0, s, g.st);
   const MGArgs a = make_args(m, m32, g, x, b, it, false);
   hipLaunchKernelGGL(k_mg_init, dim3(xcd_grid(g.nchunks)), dim3(BLOCK), 0, s, a);
-  enqueue_vcycle(a, s);
+  hipLaunchKernelGGL(k_mg_test, dim3(1), dim3(BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mg_vcycle(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+                            hipStream_t s) {
+  if (!valid(g)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mg_reopen, dim3(1), dim3(1), 0, s, g.st);
+  enqueue_vcycle(make_args(m, m32, g, x, b, it, false), s);
   return hipGetLastError();
 }
 

@@ -21,10 +21,10 @@
 
 namespace mpic {
 
-constexpr int MG_DIRECT_MAX = 64;   // unknowns of a level solved by its dense inverse
+constexpr int MG_DIRECT_MAX = 512;  // unknowns of a level solved by its dense inverse
 constexpr float MG_OMEGA = 0.8f;    // Jacobi damping
 constexpr int MG_MAX_LEVELS = 12;
-constexpr int MG_CB_MAX = 4096;     // levels at most this large run in one workgroup
+constexpr int MG_CB_MAX = 512;      // levels at most this large run in one workgroup
 constexpr int MG_MAX_RECHECKS = 3;  // restarts from the true residual per solve
 
 // One coarse level: interior points of a lattice of (cx, cy, cz) cells,
@@ -41,7 +41,7 @@ struct MGHost {
   int nx = 0, ny = 0, nz = 0;         // level 0 cells (the mesh)
   std::vector<float> dinv0;           // [nn] level 0 diagonal inverse
   std::vector<MGHostLevel> levels;    // levels 1..L; the last is solved directly
-  std::vector<float> coarse_inv;      // [n_L][n_L] inverse of the coarsest operator
+  std::vector<float> coarse_inv;      // inverse of the coarsest operator, column-major ([j][i] = (i, j))
 };
 
 // Builds the hierarchy from the FP64 stiffness matrix in CSR (rows of the

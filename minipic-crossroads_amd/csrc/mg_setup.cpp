@@ -232,8 +232,11 @@ bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const
     why = "the coarsest operator is not positive definite";
     return false;
   }
+  // column-major: the device's thread i reads element (i, j) at j * nc + i, so
+  // the threads of a wavefront read consecutive words
   out.coarse_inv.resize(dense.size());
-  for (size_t i = 0; i < dense.size(); ++i) out.coarse_inv[i] = (float)dense[i];
+  for (int64_t i = 0; i < nc; ++i)
+    for (int64_t j = 0; j < nc; ++j) out.coarse_inv[(size_t)j * nc + i] = (float)dense[(size_t)i * nc + j];
   // level 0's diagonal inverse (all nodes; the Dirichlet rows are identity rows)
   out.dinv0.assign(nn, 1.0f);
   for (int32_t r = 0; r < nn; ++r)

@@ -437,10 +437,14 @@ hipError_t launch_mg_start(const DevMesh& m, const Mesh32& m32, MGBuffers& g, do
 // update, then the V-cycle of iteration it + 1 (which first tests convergence).
 hipError_t launch_mg_iter(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
                           bool restart, hipStream_t s);
-// After a stop: reopens the solve, recomputes r = b - A x in FP64 and runs the
-// V-cycle of iteration it, whose first kernel tests the true residual.
+// After a stop: reopens the solve, recomputes r = b - A x in FP64 and tests it
+// (the state is done again if it meets the tolerance or the limit).
 hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
                              hipStream_t s);
+// The V-cycle of iteration it on the current residual (after a failed recheck:
+// the iteration then restarts from it).
+hipError_t launch_mg_vcycle(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+                            hipStream_t s);
 
 struct SortBuffers {
   int32_t* count;    // [ne+1]

@@ -115,7 +115,7 @@ struct VCycle {
     if (l == lev.size() - 1) {
       std::vector<float> x(n, 0.0f);
       for (size_t i = 0; i < n; ++i)
-        for (size_t j = 0; j < n; ++j) x[i] += h.coarse_inv[i * n + j] * b[j];
+        for (size_t j = 0; j < n; ++j) x[i] += h.coarse_inv[j * n + i] * b[j];  // column-major
       return x;
     }
     const auto& D = dinv(l);

@@ -37,11 +37,13 @@ def solve_device(mp, args, kw, b, multigrid=1, phi0=None):
     return ok, it, rr, phi, levels, why
 
 
+# (mesh, options, levels): levels coarsen until at most 512 unknowns remain,
+# which the coarsest level solves with its dense inverse
 CASES = [
-    ((16, 16, 16, 6), {}, 3),
-    ((16, 16, 8, 5), {"mapping": 2}, 3),
-    ((16, 16, 16, 6), {"jitter": 0.2, "jitter_seed": 3}, 3),
-    ((32, 32, 32, 6), {}, 4),
+    ((16, 16, 16, 6), {}, 2),
+    ((16, 16, 8, 5), {"mapping": 2}, 2),
+    ((16, 16, 16, 6), {"jitter": 0.2, "jitter_seed": 3}, 2),
+    ((32, 32, 32, 6), {}, 3),
 ]
 
 
@@ -128,13 +130,13 @@ def test_multigrid_not_applicable(mp):
 
 
 def test_multigrid_c2_mesh(mp, oracle):
-    """The C2 mesh (Kuhn 64^3, 274,625 nodes): five levels, the true residual
+    """The C2 mesh (Kuhn 64^3, 274,625 nodes): four levels, the true residual
     below 1e-10 |b| in at most 25 iterations."""
     say("multigrid C2 mesh")
     args = (64, 64, 64, 6)
     om = oracle.Mesh(*args)
     b = rhs_for(om, 1)
     ok, it, rr, phi, lv, why = solve_device(mp, args, {}, b)
-    assert lv == 5, why
+    assert lv == 4, why
     assert ok and it <= 25, (ok, it, rr)
     assert np.linalg.norm(b - om.spmv(phi)) / np.linalg.norm(b) < 1e-10
