@@ -83,12 +83,6 @@ int mg_prepare(mpic_ctx* c) {
   g.ny = h.ny;
   g.nz = h.nz;
   g.nlev = (int)h.levels.size();
-  g.cb = g.nlev;
-  for (int l = 1; l <= g.nlev; ++l)
-    if (h.levels[l - 1].n <= MG_CB_MAX) {
-      g.cb = l;
-      break;
-    }
   g.nchunks = c->cg.nchunks;
   const int32_t nn = c->m.nn;
   int rc;

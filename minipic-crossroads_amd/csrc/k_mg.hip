@@ -11,13 +11,15 @@
 //   k_mg_update  alpha = rho / p.q; x += alpha p; r -= alpha q; the V-cycle's
 //                right-hand side b0 = (float) r and first smoothing step
 //                x0 = omega D^-1 b0; partials of r.r
-//   V-cycle      k_mg_resid0 (convergence test, then t0 = b0 - A x0), restriction
-//                to level 1, the large coarse levels (residual, restriction) one
-//                launch each, k_mg_coarse (the small levels and the coarsest
-//                dense solve in one workgroup), then back up: prolongation
-//                t = x + P z_coarse and the second smoothing step
-//                z = t + omega D^-1 (b - A t) per level; level 0's last kernel
-//                writes z (into b0) and the partials of r.z.
+//   V-cycle      k_mg_resid0 (convergence test, then t0 = b0 - A x0); per coarse
+//                level one k_mg_rr (restriction b = P^T t, first smoothing step
+//                x = omega D^-1 b, residual t = b - A x, over 3-D tiles with
+//                their halo in LDS); k_mg_cs (the coarsest level's restriction
+//                and dense solve, rows spread over workgroups); back up, per
+//                level one k_mg_pp (y = x + P z_coarse and the second smoothing
+//                step z = y + omega D^-1 (b - A y), into b); level 0's
+//                prolongation, then its smoothing step, which writes z (into b0)
+//                and the partials of r.z.
 // The V-cycle is one damped-Jacobi sweep before and one after the coarse
 // correction, restriction = P^T of the trilinear interpolation P: a symmetric
 // preconditioner (up to FP32 rounding).  Every launch after the stop returns at
@@ -29,8 +31,6 @@
 
 namespace mpic {
 namespace {
-
-constexpr int CB_THREADS = 1024;  // the coarse-block kernel's single workgroup
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -79,14 +79,6 @@ __device__ __forceinline__ bool cinterior(const MGLevelDev& L, int i, int j, int
 // Level 0 points: every node, numbered as the brick generator does.
 __device__ __forceinline__ int nidx(const MGBuffers& g, int i, int j, int k) {
   return (i * (g.ny + 1) + j) * (g.nz + 1) + k;
-}
-
-// Row sums of the 27-entry ELL of a coarse level.
-__device__ __forceinline__ float crow(const MGLevelDev& L, int u, const float* __restrict__ v) {
-  float acc = 0.0f;
-#pragma unroll 9
-  for (int e = 0; e < 27; ++e) acc += L.val[e * L.n + u] * v[L.col[e * L.n + u]];
-  return acc;
 }
 
 // Level 0 row sum over the FP32 sliced ELL (diagonal first), four entries per batch.
@@ -143,36 +135,33 @@ struct MGArgs {
   int restart;
 };
 
-// ---- restriction and prolongation (shared by the multi-workgroup and the
-// single-workgroup kernels) ----
+// ---- restriction, prolongation and the tiles of the coarse levels ----
 
-// Coarse point u of level lc from level lc - 1's t: b = P^T t, x = omega D^-1 b.
-__device__ __forceinline__ void restrict_point(const MGArgs& a, int lc, int u) {
-  const MGLevelDev& C = a.g.lev[lc];
-  int I, J, K;
-  cdecode(C, u, I, J, K);
+// P^T t of the finer level at point (I, J, K) of a coarse level: full
+// weighting of the 27 fine points 2I - 1..2I + 1 (per direction), all interior.
+// The finer level is level 0 (t0 on the node lattice) when F.n == 0, else F.
+__device__ __forceinline__ float restrict_at(const MGBuffers& g, const MGLevelDev& F, int I, int J, int K) {
   float s = 0.0f;
-  if (lc == 1) {
-    const float* __restrict__ t = a.g.t0;
+  if (F.n == 0) {
+    const float* __restrict__ t = g.t0;
 #pragma unroll
     for (int di = -1; di <= 1; ++di)
 #pragma unroll
       for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
         for (int dk = -1; dk <= 1; ++dk)
-          s += (w1(di) * w1(dj) * w1(dk)) * t[nidx(a.g, 2 * I + di, 2 * J + dj, 2 * K + dk)];
+          s += (w1(di) * w1(dj) * w1(dk)) * t[nidx(g, 2 * I + di, 2 * J + dj, 2 * K + dk)];
   } else {
-    const MGLevelDev& F = a.g.lev[lc - 1];
+    const float* __restrict__ t = F.t;
 #pragma unroll
     for (int di = -1; di <= 1; ++di)
 #pragma unroll
       for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
         for (int dk = -1; dk <= 1; ++dk)
-          s += (w1(di) * w1(dj) * w1(dk)) * F.t[cidx(F, 2 * I + di, 2 * J + dj, 2 * K + dk)];
+          s += (w1(di) * w1(dj) * w1(dk)) * t[cidx(F, 2 * I + di, 2 * J + dj, 2 * K + dk)];
   }
-  C.b[u] = s;
-  C.x[u] = MG_OMEGA * C.dinv[u] * s;
+  return s;
 }
 
 // P z of level lc + 1 at point (i, j, k) of level lc (z is in that level's b).
@@ -183,6 +172,56 @@ __device__ __forceinline__ float prolong_at(const MGLevelDev& C, int i, int j, i
       for (int K = k >> 1; K <= (k + 1) >> 1; ++K)
         if (cinterior(C, I, J, K)) s += (w1(i - 2 * I) * w1(j - 2 * J) * w1(k - 2 * K)) * C.b[cidx(C, I, J, K)];
   return s;
+}
+
+// A coarse level in tiles of TX x TY x TZ points, one per workgroup (one point
+// per thread), each with a one-point halo in LDS: a row's 27 stencil columns are
+// its halo neighbours.  Halo point (hi, hj, hk) is lattice point
+// (i0 + hi, j0 + hj, k0 + hk); the tile's own points have h* in 1..T*.
+constexpr int TX = 4, TY = 8, TZ = 8;
+constexpr int HY = TY + 2, HZ = TZ + 2, HALO = (TX + 2) * HY * HZ;
+static_assert(TX * TY * TZ == BLOCK, "one tile point per thread");
+
+__host__ __device__ inline int tiles_along(int cells, int t) { return (cells - 1 + t - 1) / t; }
+inline unsigned tiles_of(const MGLevelDev& L) {
+  return (unsigned)(tiles_along(L.cx, TX) * tiles_along(L.cy, TY) * tiles_along(L.cz, TZ));
+}
+
+struct TileAt {
+  int i0, j0, k0;  // lattice point of halo (0, 0, 0)
+  int hc;          // this thread's own halo index
+  int i, j, k;     // this thread's own lattice point
+};
+__device__ __forceinline__ TileAt tile_at(const MGLevelDev& L) {
+  const int ty = tiles_along(L.cy, TY), tz = tiles_along(L.cz, TZ);
+  const int b = blockIdx.x;
+  TileAt T;
+  T.i0 = (b / (ty * tz)) * TX;
+  T.j0 = ((b / tz) % ty) * TY;
+  T.k0 = (b % tz) * TZ;
+  const int t = threadIdx.x;
+  const int hi = t / (TY * TZ) + 1, hj = (t / TZ) % TY + 1, hk = t % TZ + 1;
+  T.hc = (hi * HY + hj) * HZ + hk;
+  T.i = T.i0 + hi;
+  T.j = T.j0 + hj;
+  T.k = T.k0 + hk;
+  return T;
+}
+__device__ __forceinline__ void halo_point(const TileAt& T, int h, int& i, int& j, int& k) {
+  i = T.i0 + h / (HY * HZ);
+  j = T.j0 + (h / HZ) % HY;
+  k = T.k0 + h % HZ;
+}
+// Halo offset of stencil entry e = (di + 1) 9 + (dj + 1) 3 + (dk + 1).
+__device__ __forceinline__ int halo_off(int e) { return ((e / 9 - 1) * HY + (e / 3) % 3 - 1) * HZ + e % 3 - 1; }
+
+// Row sum of the tile's own point over its 27 stencil entries, the operand
+// from the halo (zero outside the level, where the entry is zero too).
+__device__ __forceinline__ float tile_row(const MGLevelDev& L, int u, const float* hv, int hc) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc += L.val[e * L.n + u] * hv[hc + halo_off(e)];
+  return acc;
 }
 
 // ---- kernels ----
@@ -267,87 +306,114 @@ __global__ __launch_bounds__(BLOCK) void k_mg_resid0(MGArgs a) {
   if (i < a.m.nn) a.g.t0[i] = a.g.b0[i] - row32(a.m, a.val32, i, a.g.x0);
 }
 
-// Coarse level l (below the single-workgroup levels): t = b - A x.
-__global__ __launch_bounds__(BLOCK) void k_mg_resid(MGArgs a, int l) {
-  if (a.g.st->done) return;
-  const MGLevelDev& L = a.g.lev[l];
-  const int u = blockIdx.x * BLOCK + threadIdx.x;
-  if (u < L.n) L.t[u] = L.b[u] - crow(L, u, L.x);
-}
+// The coarse-level kernels take their levels by value (the descriptors'
+// device pointers): a launch is a short chain of dependent memory accesses.
 
-// Restriction to coarse level lc.
-__global__ __launch_bounds__(BLOCK) void k_mg_restrict(MGArgs a, int lc) {
-  if (a.g.st->done) return;
-  const int u = blockIdx.x * BLOCK + threadIdx.x;
-  if (u < a.g.lev[lc].n) restrict_point(a, lc, u);
-}
-
-// Prolongation to coarse level l (from l + 1) and its second smoothing step,
-// as two launches: t = x + P z, then z = t + omega D^-1 (b - A t) into b.
-__global__ __launch_bounds__(BLOCK) void k_mg_prolong(MGArgs a, int l) {
-  if (a.g.st->done) return;
-  const MGLevelDev& L = a.g.lev[l];
-  const int u = blockIdx.x * BLOCK + threadIdx.x;
-  if (u >= L.n) return;
-  int i, j, k;
-  cdecode(L, u, i, j, k);
-  L.t[u] = L.x[u] + prolong_at(a.g.lev[l + 1], i, j, k);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_mg_post(MGArgs a, int l) {
-  if (a.g.st->done) return;
-  const MGLevelDev& L = a.g.lev[l];
-  const int u = blockIdx.x * BLOCK + threadIdx.x;
-  if (u < L.n) L.b[u] = L.t[u] + MG_OMEGA * L.dinv[u] * (L.b[u] - crow(L, u, L.t));
-}
-
-// Levels cb..nlev in one workgroup: down (residual, restriction), the
-// coarsest dense solve, up (prolongation, second smoothing step).
-__global__ __launch_bounds__(CB_THREADS) void k_mg_coarse(MGArgs a) {
-  if (wg_done(a.g.st)) return;
-  const int tid = threadIdx.x;
-  const int nlev = a.g.nlev;
-  for (int l = a.g.cb; l < nlev; ++l) {
-    const MGLevelDev& L = a.g.lev[l];
-    for (int u = tid; u < L.n; u += CB_THREADS) L.t[u] = L.b[u] - crow(L, u, L.x);
-    __syncthreads();
-    const int nc = a.g.lev[l + 1].n;
-    for (int u = tid; u < nc; u += CB_THREADS) restrict_point(a, l + 1, u);
-    __syncthreads();
-  }
-  {
-    const MGLevelDev& C = a.g.lev[nlev];
-    const int n = C.n;  // <= MG_DIRECT_MAX <= CB_THREADS: one row of the dense inverse per thread
-    float z = 0.0f;
-    if (tid < n) {
-      // the inverse is stored column-major: (tid, j) at j * n + tid
-      const float* __restrict__ col = a.g.cinv + tid;
-      const float* __restrict__ b = C.b;
-      int j = 0;
-      float z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
-      for (; j + 4 <= n; j += 4) {
-        z += col[(size_t)j * n] * b[j];
-        z1 += col[(size_t)(j + 1) * n] * b[j + 1];
-        z2 += col[(size_t)(j + 2) * n] * b[j + 2];
-        z3 += col[(size_t)(j + 3) * n] * b[j + 3];
-      }
-      for (; j < n; ++j) z += col[(size_t)j * n] * b[j];
-      z = (z + z1) + (z2 + z3);
+// Coarse level L on the way down, one tile per workgroup: the restriction
+// b = P^T t of the finer level F (level 0 when F.n == 0) and the first
+// smoothing step x = omega D^-1 b over the tile and its halo (into LDS), then
+// the residual t = b - A x of the tile's points.
+__global__ __launch_bounds__(BLOCK) void k_mg_rr(MGArgs a, MGLevelDev L, MGLevelDev F) {
+  __shared__ float xs[HALO], bs[HALO];
+  if (a.g.st->done) return;  // set by an earlier launch: the same for every wave
+  const TileAt T = tile_at(L);
+  for (int h = threadIdx.x; h < HALO; h += BLOCK) {
+    int i, j, k;
+    halo_point(T, h, i, j, k);
+    float b = 0.0f, x = 0.0f;
+    if (cinterior(L, i, j, k)) {
+      b = restrict_at(a.g, F, i, j, k);
+      x = MG_OMEGA * L.dinv[cidx(L, i, j, k)] * b;
     }
-    __syncthreads();
-    if (tid < n) C.b[tid] = z;
-    __syncthreads();
+    bs[h] = b;
+    xs[h] = x;
   }
-  for (int l = nlev - 1; l >= a.g.cb; --l) {
-    const MGLevelDev& L = a.g.lev[l];
-    for (int u = tid; u < L.n; u += CB_THREADS) {
-      int i, j, k;
-      cdecode(L, u, i, j, k);
-      L.t[u] = L.x[u] + prolong_at(a.g.lev[l + 1], i, j, k);
+  __syncthreads();
+  if (!cinterior(L, T.i, T.j, T.k)) return;
+  const int u = cidx(L, T.i, T.j, T.k);
+  float acc = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc += L.val[e * L.n + u] * xs[T.hc + halo_off(e)];
+  const float b = bs[T.hc];
+  L.b[u] = b;
+  L.x[u] = xs[T.hc];
+  L.t[u] = b - acc;
+}
+
+// Coarse level L on the way up, one tile per workgroup: the corrected iterate
+// y = x + P z of the coarser level C over the tile and its halo (into LDS),
+// then the second smoothing step z = y + omega D^-1 (b - A y) of the tile's
+// points, into b.
+__global__ __launch_bounds__(BLOCK) void k_mg_pp(MGArgs a, MGLevelDev L, MGLevelDev C) {
+  __shared__ float ys[HALO];
+  if (a.g.st->done) return;
+  const TileAt T = tile_at(L);
+  const bool own = cinterior(L, T.i, T.j, T.k);
+  const int u = own ? cidx(L, T.i, T.j, T.k) : 0;
+  const float bu = own ? L.b[u] : 0.0f, du = own ? L.dinv[u] : 0.0f;
+  for (int h = threadIdx.x; h < HALO; h += BLOCK) {
+    int i, j, k;
+    halo_point(T, h, i, j, k);
+    ys[h] = cinterior(L, i, j, k) ? L.x[cidx(L, i, j, k)] + prolong_at(C, i, j, k) : 0.0f;
+  }
+  __syncthreads();
+  if (!own) return;
+  float acc = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc += L.val[e * L.n + u] * ys[T.hc + halo_off(e)];
+  const float y = ys[T.hc];
+  L.b[u] = y + MG_OMEGA * du * (bu - acc);
+}
+
+// The coarsest level C: every workgroup restricts the whole level (at most
+// MG_DIRECT_MAX points) from the finer level F into LDS, then forms CS_ROWS
+// rows of z = A^-1 b from the dense inverse (stored column-major), the columns
+// split in CS_PARTS contiguous parts summed in a fixed order.
+constexpr int CS_ROWS = 16, CS_PARTS = BLOCK / CS_ROWS, CS_PER = MG_DIRECT_MAX / CS_PARTS;
+static_assert(CS_PER * CS_PARTS == MG_DIRECT_MAX, "column parts");
+__global__ __launch_bounds__(BLOCK) void k_mg_cs(MGArgs a, MGLevelDev C, MGLevelDev F) {
+  __shared__ float bsh[MG_DIRECT_MAX];
+  __shared__ float part[CS_PARTS][CS_ROWS];
+  if (a.g.st->done) return;
+  const int n = C.n;  // <= MG_DIRECT_MAX
+  const int rr = threadIdx.x % CS_ROWS, p = threadIdx.x / CS_ROWS;
+  const int row = blockIdx.x * CS_ROWS + rr;
+  const int per = (n + CS_PARTS - 1) / CS_PARTS;  // <= CS_PER
+  const int j0 = p * per, j1 = min(n, j0 + per);
+  float m[CS_PER];
+  const float* __restrict__ col = a.g.cinv + row;
+#pragma unroll
+  for (int q = 0; q < CS_PER; ++q) m[q] = (row < n && j0 + q < j1) ? col[(size_t)(j0 + q) * n] : 0.0f;
+  for (int u = threadIdx.x; u < MG_DIRECT_MAX; u += BLOCK) {
+    float b = 0.0f;  // zero past n: the clamped reads below multiply it by zero
+    if (u < n) {
+      int I, J, K;
+      cdecode(C, u, I, J, K);
+      b = restrict_at(a.g, F, I, J, K);
     }
-    __syncthreads();
-    for (int u = tid; u < L.n; u += CB_THREADS) L.b[u] = L.t[u] + MG_OMEGA * L.dinv[u] * (L.b[u] - crow(L, u, L.t));
-    __syncthreads();
+    bsh[u] = b;
+  }
+  __syncthreads();
+  float z = 0.0f, z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
+#pragma unroll
+  for (int q = 0; q < CS_PER; q += 4) {
+    // entries past j1 are zero and read a valid LDS word (clamped)
+    z += m[q] * bsh[min(j0 + q, MG_DIRECT_MAX - 1)];
+    z1 += m[q + 1] * bsh[min(j0 + q + 1, MG_DIRECT_MAX - 1)];
+    z2 += m[q + 2] * bsh[min(j0 + q + 2, MG_DIRECT_MAX - 1)];
+    z3 += m[q + 3] * bsh[min(j0 + q + 3, MG_DIRECT_MAX - 1)];
+  }
+  part[p][rr] = (z + z1) + (z2 + z3);
+  __syncthreads();
+  if (threadIdx.x < CS_ROWS && row < n) {
+    float w[CS_PARTS];
+#pragma unroll
+    for (int q = 0; q < CS_PARTS; ++q) w[q] = part[q][rr];
+#pragma unroll
+    for (int step = 1; step < CS_PARTS; step *= 2)
+#pragma unroll
+      for (int q = 0; q + step < CS_PARTS; q += 2 * step) w[q] += w[q + step];
+    C.b[row] = w[0];
   }
 }
 
@@ -361,7 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_mg_prolong0(MGArgs a) {
   const int nz1 = a.g.nz + 1, ny1 = a.g.ny + 1;
   const int k = u % nz1, j = (u / nz1) % ny1, i = u / (nz1 * ny1);
   float y = a.g.x0[u];
-  if (i > 0 && i < a.g.nx && j > 0 && j < a.g.ny && k > 0 && k < a.g.nz) y += prolong_at(a.g.lev[1], i, j, k);
+  if (i > 0 && i < a.g.nx && j > 0 && j < a.g.ny && k > 0 && k < a.g.nz) y += prolong_at(a.g.host_lev[1], i, j, k);
   a.g.t0[u] = y;
 }
 
@@ -455,29 +521,29 @@ MGArgs make_args(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, c
   return a;
 }
 
-inline unsigned blocks_for(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-
-// The V-cycle of iteration a.it (its first kernel tests the stop).
+// The V-cycle of iteration a.it (its first kernel tests the stop): level 0's
+// residual; levels 1..nlev-1 restricted with their residuals; the coarsest
+// level restricted and solved; back up, levels nlev-1..1 corrected with their
+// second smoothing step; level 0's prolongation and second smoothing step.
 void enqueue_vcycle(const MGArgs& a, hipStream_t s) {
   const MGBuffers& g = a.g;
   const dim3 grid0(xcd_grid(g.nchunks));
+  const MGLevelDev none{};  // n == 0: the finer level is level 0
   hipLaunchKernelGGL(k_mg_resid0, grid0, dim3(BLOCK), 0, s, a);
-  hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(g.host_lev[1].n)), dim3(BLOCK), 0, s, a, 1);
-  for (int l = 1; l < g.cb; ++l) {
-    hipLaunchKernelGGL(k_mg_resid, dim3(blocks_for(g.host_lev[l].n)), dim3(BLOCK), 0, s, a, l);
-    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(g.host_lev[l + 1].n)), dim3(BLOCK), 0, s, a, l + 1);
-  }
-  hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CB_THREADS), 0, s, a);
-  for (int l = g.cb - 1; l >= 1; --l) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(g.host_lev[l].n)), dim3(BLOCK), 0, s, a, l);
-    hipLaunchKernelGGL(k_mg_post, dim3(blocks_for(g.host_lev[l].n)), dim3(BLOCK), 0, s, a, l);
-  }
+  for (int l = 1; l < g.nlev; ++l)
+    hipLaunchKernelGGL(k_mg_rr, dim3(tiles_of(g.host_lev[l])), dim3(BLOCK), 0, s, a, g.host_lev[l],
+                       l == 1 ? none : g.host_lev[l - 1]);
+  const unsigned ncs = (unsigned)((g.host_lev[g.nlev].n + CS_ROWS - 1) / CS_ROWS);
+  hipLaunchKernelGGL(k_mg_cs, dim3(ncs), dim3(BLOCK), 0, s, a, g.host_lev[g.nlev],
+                     g.nlev == 1 ? none : g.host_lev[g.nlev - 1]);
+  for (int l = g.nlev - 1; l >= 1; --l)
+    hipLaunchKernelGGL(k_mg_pp, dim3(tiles_of(g.host_lev[l])), dim3(BLOCK), 0, s, a, g.host_lev[l], g.host_lev[l + 1]);
   hipLaunchKernelGGL(k_mg_prolong0, grid0, dim3(BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_mg_post0, grid0, dim3(BLOCK), 0, s, a);
 }
 
 bool valid(const MGBuffers& g) {
-  return g.lev && g.nlev >= 1 && g.nlev <= MG_MAX_LEVELS && g.cb >= 1 && g.cb <= g.nlev && g.dinv0 && g.cinv &&
+  return g.lev && g.nlev >= 1 && g.nlev <= MG_MAX_LEVELS && g.dinv0 && g.cinv &&
          g.b0 && g.x0 && g.t0 && g.r && g.q && g.p[0] && g.p[1] && g.part_bb && g.part_rr && g.part_rz &&
          g.part_pq && g.rz_hist && g.st && g.nchunks > 0 && g.host_lev[g.nlev].n <= MG_DIRECT_MAX;
 }

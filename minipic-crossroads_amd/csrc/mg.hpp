@@ -24,7 +24,6 @@ namespace mpic {
 constexpr int MG_DIRECT_MAX = 512;  // unknowns of a level solved by its dense inverse
 constexpr float MG_OMEGA = 0.8f;    // Jacobi damping
 constexpr int MG_MAX_LEVELS = 12;
-constexpr int MG_CB_MAX = 512;      // levels at most this large run in one workgroup
 constexpr int MG_MAX_RECHECKS = 3;  // restarts from the true residual per solve
 
 // One coarse level: interior points of a lattice of (cx, cy, cz) cells,

@@ -415,7 +415,6 @@ struct MGState {
 };
 struct MGBuffers {
   int nlev = 0;             // coarse levels 1..nlev; the last is solved by its dense inverse
-  int cb = 0;               // first level of the single-workgroup kernel (levels cb..nlev)
   int nx = 0, ny = 0, nz = 0;
   int nchunks = 0;          // level 0 row chunks of BLOCK rows (partials are indexed by chunk)
   const MGLevelDev* lev = nullptr;            // device [nlev + 1] (entry 0 unused)
