@@ -75,13 +75,15 @@ def mg_bytes_per_iter(cells, n, nnz):
       level 0 of the V-cycle: two FP32 SpMVs (8 nnz each) with b0, x0, t0, dinv,
         r and z read or written once (32 n), restriction and prolongation
         (t0, x0 read, t0 written: 12 n);
-      coarse levels of n_l unknowns: two 27-entry FP32 SpMVs (2 x 216 n_l) and
-        their vectors (40 n_l); the coarsest level's dense inverse (4 n_c^2)."""
+      coarse levels of n_l unknowns: the 27-point stencil's FP32 values read
+        twice (2 x 108 n_l; the columns are the tile's halo in LDS) and the
+        level's vectors (restriction input, b, x, t, D^-1 and the coarser z:
+        40 n_l); the coarsest level's dense inverse (4 n_c^2)."""
     cg = 12 * nnz + 28 * n + 56 * n
     sizes = mg_levels_sizes(cells)
     vc = 16 * nnz + 44 * n
     for nl in sizes[:-1]:
-        vc += 2 * 216 * nl + 40 * nl
+        vc += 2 * 108 * nl + 40 * nl
     if sizes:
         vc += 4 * sizes[-1] ** 2 + 16 * sizes[-1]
     return cg, vc

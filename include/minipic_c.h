@@ -87,7 +87,7 @@ typedef struct {
   double species_q[MPIC_MAX_SPECIES];
   double species_m[MPIC_MAX_SPECIES];
   int solver;                            /* MPIC_SOLVER_*: the solve of mpic_solve and mpic_time_step (FP64 form;
-                                            the precision form always runs its refinement CG) */
+                                            the precision form always runs its own CG, see multigrid) */
   int persistent_solve;                  /* 1: a solve is one cooperative launch when its grid fits on the GPU
                                             (same results, but slower on MI355X: its grid-wide waits cost more
                                             than a kernel boundary); 0 (default): one launch per iteration */

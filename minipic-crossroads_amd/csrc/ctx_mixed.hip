@@ -89,17 +89,15 @@ int mg_prepare(mpic_ctx* c) {
   for (int l = 1; l <= g.nlev; ++l) {
     const MGHostLevel& H = h.levels[l - 1];
     MGLevelDev& L = g.host_lev[l];
-    int32_t* col;
     float *val, *dinv;
     L.cx = H.cx;
     L.cy = H.cy;
     L.cz = H.cz;
     L.n = H.n;
-    if ((rc = dupload(c, &col, H.col.data(), H.col.size())) || (rc = dupload(c, &val, H.val.data(), H.val.size())) ||
+    if ((rc = dupload(c, &val, H.val.data(), H.val.size())) ||
         (rc = dupload(c, &dinv, H.dinv.data(), H.dinv.size())) || (rc = dalloc(c, &L.b, (size_t)H.n)) ||
         (rc = dalloc(c, &L.x, (size_t)H.n)) || (rc = dalloc(c, &L.t, (size_t)H.n)))
       return rc;
-    L.col = col;
     L.val = val;
     L.dinv = dinv;
   }

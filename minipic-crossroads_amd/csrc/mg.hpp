@@ -28,7 +28,11 @@ constexpr int MG_MAX_RECHECKS = 3;  // restarts from the true residual per solve
 
 // One coarse level: interior points of a lattice of (cx, cy, cz) cells,
 // numbered ((i - 1) (cy - 1) + (j - 1)) (cz - 1) + (k - 1); its operator as a
-// 27-entry ELL, column-major ([27][n]), padding (col = row, val = 0).
+// 27-point stencil, column-major ([27][n]), entry (di + 1) 9 + (dj + 1) 3 +
+// (dk + 1) coupling the point to its neighbour at offset (di, dj, dk); zero
+// where that neighbour is not an interior point.  The columns (col; the row
+// itself where the value is zero) serve the host-side checks; the device
+// kernels take a row's neighbours from the stencil offsets.
 struct MGHostLevel {
   int cx = 0, cy = 0, cz = 0;
   int32_t n = 0;

@@ -404,8 +404,7 @@ hipError_t launch_ir_persist(const DevMesh& m, const Mesh32& m32, IRBuffers& c, 
 // prolonged correction).
 struct MGLevelDev {
   int cx, cy, cz, n;
-  const int32_t* col;
-  const float* val;
+  const float* val;  // 27-point stencil, [27][n], entry (di + 1) 9 + (dj + 1) 3 + (dk + 1)
   const float* dinv;
   float *b, *x, *t;
 };
