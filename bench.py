@@ -54,12 +54,12 @@ def cg_bytes_per_iter(n, nnz, word):
 
 def mg_levels_sizes(cells):
     """Unknowns of the coarse levels of the multigrid solve on a cubic brick of
-    `cells` cells per direction: halve while even and >= 4; the level with at
-    most 512 unknowns is the coarsest (dense inverse)."""
+    `cells` cells per direction: c cells coarsen to (c + 1) // 2 while c >= 3;
+    the level with at most 512 unknowns is the coarsest (dense inverse)."""
     sizes = []
     c = cells
-    while c % 2 == 0 and c >= 4:
-        c //= 2
+    while c >= 3:
+        c = (c + 1) // 2
         sizes.append((c - 1) ** 3)
         if sizes[-1] <= 512:
             break

@@ -138,28 +138,38 @@ struct MGArgs {
 // ---- restriction, prolongation and the tiles of the coarse levels ----
 
 // P^T t of the finer level at point (I, J, K) of a coarse level: full
-// weighting of the 27 fine points 2I - 1..2I + 1 (per direction), all interior.
-// The finer level is level 0 (t0 on the node lattice) when F.n == 0, else F.
+// weighting of the fine points 2I - 1..2I + 1 (per direction) that are
+// interior: 2I + 1 is the fine wall when the fine cell count is odd, and takes
+// weight 0 (its load, at the clamped point 2I, stays in bounds; all 27 loads
+// are issued unconditionally).  The finer level is level 0 (t0 on the node
+// lattice) when F.n == 0, else F.
 __device__ __forceinline__ float restrict_at(const MGBuffers& g, const MGLevelDev& F, int I, int J, int K) {
   float s = 0.0f;
-  if (F.n == 0) {
+  const bool top = F.n == 0;
+  const bool okx = 2 * I + 1 < (top ? g.nx : F.cx), oky = 2 * J + 1 < (top ? g.ny : F.cy),
+             okz = 2 * K + 1 < (top ? g.nz : F.cz);
+  const float hx = okx ? 0.5f : 0.0f, hy = oky ? 0.5f : 0.0f, hz = okz ? 0.5f : 0.0f;
+  const int ux = okx ? 1 : 0, uy = oky ? 1 : 0, uz = okz ? 1 : 0;
+  const float wx[3] = {0.5f, 1.0f, hx}, wy[3] = {0.5f, 1.0f, hy}, wz[3] = {0.5f, 1.0f, hz};
+  const int ox[3] = {-1, 0, ux}, oy[3] = {-1, 0, uy}, oz[3] = {-1, 0, uz};
+  if (top) {
     const float* __restrict__ t = g.t0;
 #pragma unroll
-    for (int di = -1; di <= 1; ++di)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int dj = -1; dj <= 1; ++dj)
+      for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int dk = -1; dk <= 1; ++dk)
-          s += (w1(di) * w1(dj) * w1(dk)) * t[nidx(g, 2 * I + di, 2 * J + dj, 2 * K + dk)];
+        for (int c = 0; c < 3; ++c)
+          s += (wx[a] * wy[b] * wz[c]) * t[nidx(g, 2 * I + ox[a], 2 * J + oy[b], 2 * K + oz[c])];
   } else {
     const float* __restrict__ t = F.t;
 #pragma unroll
-    for (int di = -1; di <= 1; ++di)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int dj = -1; dj <= 1; ++dj)
+      for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int dk = -1; dk <= 1; ++dk)
-          s += (w1(di) * w1(dj) * w1(dk)) * t[cidx(F, 2 * I + di, 2 * J + dj, 2 * K + dk)];
+        for (int c = 0; c < 3; ++c)
+          s += (wx[a] * wy[b] * wz[c]) * t[cidx(F, 2 * I + ox[a], 2 * J + oy[b], 2 * K + oz[c])];
   }
   return s;
 }

@@ -1,16 +1,19 @@
 // Host setup of the multigrid preconditioner of the precision form's solve
 // (mg.hpp).  Levels are nested node lattices of the brick mesh: level 0 is the
 // mesh itself (nodes (i, j, k), 0 <= i <= nx, ..., numbered as the brick
-// generator numbers them, mesh.cpp:52-54), level l + 1 has half the cells of
-// level l in every direction and keeps only its interior points as unknowns
-// (Dirichlet walls are zero at every level).
+// generator numbers them, mesh.cpp:52-54); level l + 1 keeps every second
+// point of level l: a level of c cells per direction has interior points
+// 1..c - 1, and its coarse level has floor((c + 1) / 2) cells, coarse point I
+// sitting on fine point 2I (an odd c leaves a last coarse cell of one fine
+// cell).  Only interior points are unknowns (Dirichlet walls are zero at every
+// level).
 //
-// Interpolation P from level l + 1 to level l is trilinear on the lattice: a
-// fine point (i, j, k) takes, per direction, the coarse point i/2 (i even,
-// weight 1) or the two points (i -+ 1)/2 (i odd, weight 1/2 each); coarse
-// points on the walls contribute nothing.  Restriction is P^T (full weighting:
-// coarse point I gathers fine points 2I - 1, 2I, 2I + 1 per direction with
-// weights 1/2, 1, 1/2).  The coarse operator is the Galerkin product
+// Interpolation P from level l + 1 to level l is linear per direction on the
+// lattice: a fine point (i, j, k) takes, per direction, the coarse point i/2
+// (i even, weight 1) or the two points (i -+ 1)/2 (i odd, weight 1/2 each);
+// coarse points on the walls contribute nothing.  Restriction is P^T (full
+// weighting: coarse point I gathers the interior fine points among 2I - 1, 2I,
+// 2I + 1 per direction with weights 1/2, 1, 1/2).  The coarse operator is the Galerkin product
 // A_{l+1} = P^T A_l P, formed here in FP64 over the interior unknowns; the
 // stencil of a tet mesh on a brick lattice couples points at most one cell
 // apart in each direction, so every coarse operator is a 27-point stencil.
@@ -77,11 +80,12 @@ void galerkin(const Grid& gf, const Stencil& af, const Grid& gc, Stencil& ac) {
     for (int J = 1; J < gc.cy; ++J)
       for (int K = 1; K < gc.cz; ++K) {
         double acc[27] = {0};
-        // fine points f = 2I + d (all interior: 1 <= 2I - 1, 2I + 1 <= fine cells - 1)
+        // fine points f = 2I + d (2I + 1 is the fine wall when the fine cell count is odd)
         for (int a = -1; a <= 1; ++a)
           for (int b = -1; b <= 1; ++b)
             for (int c = -1; c <= 1; ++c) {
               const int fi = 2 * I + a, fj = 2 * J + b, fk = 2 * K + c;
+              if (!gf.interior(fi, fj, fk)) continue;
               const double wf = w1(a) * w1(b) * w1(c);
               const double* row = &af[(size_t)gf.index(fi, fj, fk) * 27];
               for (int s = 0; s < 27; ++s) {
@@ -142,8 +146,8 @@ bool spd_inverse(int n, std::vector<double>& a) {
 bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const int32_t* row_ptr, const int32_t* col,
               const double* val, MGHost& out, std::string& why) {
   out = MGHost{};
-  if (nx < 4 || ny < 4 || nz < 4 || (int64_t)(nx + 1) * (ny + 1) * (nz + 1) != nn) {
-    why = "the mesh is not a brick lattice of at least 4 cells per direction";
+  if (nx < 3 || ny < 3 || nz < 3 || (int64_t)(nx + 1) * (ny + 1) * (nz + 1) != nn) {
+    why = "the mesh is not a brick lattice of at least 3 cells per direction";
     return false;
   }
   const Grid g0{nx, ny, nz, true};
@@ -172,8 +176,8 @@ bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const
   Grid gf = g0;
   Stencil af;
   if (!stencil_from_csr(g0, row_ptr, col, val, af, why)) return false;
-  while (gf.cx % 2 == 0 && gf.cy % 2 == 0 && gf.cz % 2 == 0 && gf.cx >= 4 && gf.cy >= 4 && gf.cz >= 4) {
-    const Grid gc{gf.cx / 2, gf.cy / 2, gf.cz / 2, false};
+  while (gf.cx >= 3 && gf.cy >= 3 && gf.cz >= 3 && (int)out.levels.size() < MG_MAX_LEVELS) {
+    const Grid gc{(gf.cx + 1) / 2, (gf.cy + 1) / 2, (gf.cz + 1) / 2, false};
     Stencil ac;
     galerkin(gf, af, gc, ac);
     MGHostLevel L;
@@ -208,13 +212,10 @@ bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const
     af.swap(ac);
     if (gf.count() <= MG_DIRECT_MAX) break;
   }
-  if (out.levels.empty()) {
-    why = "the lattice cannot be coarsened (odd cell counts)";
-    return false;
-  }
   const int64_t nc = gf.count();
-  if (nc > MG_DIRECT_MAX) {
-    why = "the coarsest level has " + std::to_string(nc) + " unknowns (odd cell counts stop the coarsening)";
+  if (out.levels.empty() || nc > MG_DIRECT_MAX) {
+    why = "the coarsest level has " + std::to_string(nc) +
+          " unknowns (a direction with fewer than 3 cells stops the coarsening)";
     return false;
   }
   // dense inverse of the coarsest operator

@@ -4,14 +4,16 @@
 // For brick meshes built by the library (Kuhn and 5-tet splits, a node
 // mapping, interior jitter):
 //   1. every coarse operator equals P^T A P formed densely from the
-//      interpolation applied to unit vectors (meshes of 8 and 12 cells);
+//      interpolation applied to unit vectors (meshes of 7 to 12 cells, even
+//      and odd counts);
 //   2. conjugate gradients in FP64 preconditioned by an FP32 V-cycle built
 //      from the hierarchy -- the algorithm of the device solve: damped Jacobi
 //      (omega = 0.8) once before and once after the coarse correction, full
 //      weighting, trilinear interpolation, the dense coarsest inverse -- reach
 //      |r| < 1e-10 |b| from zero in few iterations, independent of the mesh
 //      size, where plain CG needs hundreds;
-//   3. odd cell counts and a periodic mesh are refused with a reason.
+//   3. a direction of fewer than 3 cells and a periodic mesh are refused with
+//      a reason.
 // Prints one line per case and "mg_check ok", exit 0; or the failure, exit 1.
 #include <cmath>
 #include <cstdio>
@@ -53,7 +55,8 @@ struct Lev {
 };
 inline float w1(int d) { return d == 0 ? 1.0f : 0.5f; }
 
-// coarse b = P^T r (fine level f, coarse level c)
+// coarse b = P^T r (fine level f, coarse level c; interior fine points only:
+// 2I + 1 is the fine wall when the fine cell count is odd)
 void restrict_(const Lev& f, const Lev& c, const std::vector<float>& r, std::vector<float>& b) {
   b.assign(c.count(), 0.0f);
   for (int I = 1; I < c.cx; ++I)
@@ -62,7 +65,9 @@ void restrict_(const Lev& f, const Lev& c, const std::vector<float>& r, std::vec
         float s = 0.0f;
         for (int a = -1; a <= 1; ++a)
           for (int bb = -1; bb <= 1; ++bb)
-            for (int cc = -1; cc <= 1; ++cc) s += w1(a) * w1(bb) * w1(cc) * r[f.index(2 * I + a, 2 * J + bb, 2 * K + cc)];
+            for (int cc = -1; cc <= 1; ++cc)
+              if (f.interior(2 * I + a, 2 * J + bb, 2 * K + cc))
+                s += w1(a) * w1(bb) * w1(cc) * r[f.index(2 * I + a, 2 * J + bb, 2 * K + cc)];
         b[c.index(I, J, K)] = s;
       }
 }
@@ -312,12 +317,18 @@ void refuse_case(const Case& cs, bool periodic) {
 int main() {
   galerkin_case({8, 8, 8, 6, 0, 0.0});
   galerkin_case({12, 8, 8, 5, 2, 0.0});
+  galerkin_case({9, 8, 7, 6, 0, 0.0});
+  galerkin_case({7, 7, 7, 5, 1, 0.0});
   pcg_case({16, 16, 16, 6, 0, 0.0}, 20);
   pcg_case({32, 32, 32, 6, 0, 0.0}, 20);
   pcg_case({16, 16, 16, 5, 1, 0.0}, 25);
   pcg_case({16, 16, 8, 6, 2, 0.0}, 25);
   pcg_case({16, 16, 16, 6, 0, 0.2}, 25);
-  refuse_case({9, 8, 8, 6, 0, 0.0}, false);
+  pcg_case({15, 15, 15, 6, 0, 0.0}, 25);
+  pcg_case({23, 23, 23, 6, 0, 0.0}, 25);
+  pcg_case({17, 12, 9, 5, 0, 0.2}, 25);
+  pcg_case({31, 31, 31, 6, 0, 0.0}, 25);
+  refuse_case({16, 16, 2, 6, 0, 0.0}, false);
   refuse_case({8, 8, 8, 5, 0, 0.0}, true);
   if (failures) {
     std::printf("mg_check: %d failure(s)\n", failures);

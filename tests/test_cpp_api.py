@@ -305,11 +305,12 @@ def test_multigrid_hierarchy_cpu():
     """The precision form's multigrid hierarchy on the CPU (tests/cpp/mg_check.cpp):
     Galerkin coarse operators equal to a dense P^T A P, V-cycle preconditioned
     CG iteration counts independent of the mesh size (Kuhn and 5-tet splits, a
-    node mapping, jitter), refusal of odd lattices and periodic meshes."""
+    node mapping, jitter, even and odd cell counts), refusal of a direction
+    with fewer than 3 cells and of periodic meshes."""
     if not os.path.exists(MG_CHECK):
         pytest.skip("tests/cpp not built (make -C tests/cpp)")
     out = subprocess.run([MG_CHECK], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "mg_check ok" in out.stdout
     its = [int(l.split("multigrid ")[1].split()[0]) for l in out.stdout.splitlines() if l.startswith("pcg ")]
-    assert len(its) == 5 and max(its) <= 25
+    assert len(its) == 9 and max(its) <= 25

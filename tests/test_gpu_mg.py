@@ -44,6 +44,10 @@ CASES = [
     ((16, 16, 8, 5), {"mapping": 2}, 2),
     ((16, 16, 16, 6), {"jitter": 0.2, "jitter_seed": 3}, 2),
     ((32, 32, 32, 6), {}, 3),
+    # odd cell counts: coarse point I on fine point 2I, the last coarse cell one fine cell wide
+    ((15, 15, 15, 6), {}, 2),
+    ((17, 12, 9, 5), {"jitter": 0.2, "jitter_seed": 5}, 2),
+    ((23, 23, 23, 6), {}, 3),
 ]
 
 
@@ -111,14 +115,16 @@ def test_multigrid_iteration_limit(mp, oracle):
 
 def test_multigrid_not_applicable(mp):
     say("multigrid fallbacks")
-    cases = [((9, 8, 8, 6), {}, "odd"), ((8, 8, 8, 5), {"lo": (-1, -1, -1), "hi": (1, 1, 1)}, None)]
+    cases = [((16, 16, 2, 6), {}, "3 cells"), ((8, 8, 8, 5), {"lo": (-1, -1, -1), "hi": (1, 1, 1)}, None),
+             ((9, 8, 8, 6), {}, None)]
     for args, kw, reason in cases:
         pm = mp.Mesh(*args, **kw)
         ctx = mp.Context(pm.desc, mp.default_config(form=mp.FORM_MIXED))
         ctx.reserve(1024)
         lv, why = ctx.multigrid_levels()
         if reason is None:
-            assert lv == 2, why  # the reference's 5-tet box: 8 cells -> 4 (27 unknowns, solved directly)
+            # the reference's 5-tet box: 8 cells -> 4 (27 unknowns, solved directly); 9x8x8 -> 5x4x4 (36)
+            assert lv == 2, why
         else:
             assert lv == 0 and reason in why
         ctx.close()
