@@ -47,7 +47,9 @@ def cg_bytes_per_iter(n, nnz, word):
     bytes, int32 columns and row offsets, the input vector read once and the
     output written once) plus five vector passes (p update, p.q, x update, r
     update, r.r) of `word` bytes per row.  FP64 CG: word 8 (the SpMV part is
-    12 nnz + 4 (n+1) + 16 n); FP32 inner CG: word 4."""
+    12 nnz + 4 (n+1) + 16 n); FP32 inner CG: word 4.  nnz counts the matrix's
+    nonzero entries (and the diagonal), the entries the device reads: the
+    stored exact zeros of the CSR are left out of its ELL."""
     spmv = (word + 4) * nnz + 4 * (n + 1) + 2 * word * n
     return spmv, spmv + 5 * word * n
 
@@ -183,7 +185,8 @@ def run_gpu(args, form):
             name, i, st["cg_iters"], st["ms_solve"], st["ms_deposit"], st["ms_push"], st["ms_sort"],
             st["kinetic_energy"]))
     mg_levels = ctx.multigrid_levels()[0] if form == mp.FORM_MIXED else 0
-    out = dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nnz,
+    out = dict(npart=npart, wall=wall, stats=stats, push_ms=push_ms, ne=mesh.ne, nn=mesh.nn, nnz=mesh.nonzeros(),
+               nnz_stored=mesh.nnz,
                owner_mismatch=ctx.check_ownership(), mg_levels=mg_levels, cells=n)
     ctx.close()
     return out
@@ -411,7 +414,7 @@ def main():
             "workload": "Kuhn %d^3 x6 tets (%d), %d electrons/tet (%d particles), vth 0.1, density perturbation "
                         "0.05, CG tol 1e-10, dt %g, reorder every %d steps" % (
                             args.nx, gm["ne"], args.nppe, gm["npart"], args.dt, args.sort_period),
-            "form": "mixed", "tets": gm["ne"], "nodes": gm["nn"], "nnz": gm["nnz"], "particles": gm["npart"],
+            "form": "mixed", "tets": gm["ne"], "nodes": gm["nn"], "nnz": gm["nnz"], "nnz_stored": gm["nnz_stored"], "particles": gm["npart"],
             "cg_iters_mean": sm["cg_iters_mean"], "cg_ms_per_step": sm["cg_ms_per_step"],
             "push_ms": sm["push_ms"], "sort_ms_per_step": sm["sort_ms_per_step"],
             "sort_ms_per_reorder": sm["sort_ms_per_reorder"], "field_ms_per_step": sm["field_ms_per_step"],

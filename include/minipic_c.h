@@ -307,6 +307,10 @@ typedef struct {
   int64_t nonpositive;  /* sub-steps with a non-positive exit time */
   int ke_rescaled;      /* 1: the largest kinetic-energy term outgrew the fixed-point scale set at fill;
                            the per-problem sums were redone at a safe scale and the graphs recaptured */
+  int multigrid;        /* 1: every problem solves by FP64 CG preconditioned by an FP32 multigrid V-cycle on
+                           its own hierarchy (cfg.multigrid and every mesh a brick lattice of >= 3 cells per
+                           direction); 0: FP32 CG inside FP64 refinement.  max_inner_iters then counts the
+                           preconditioned iterations. */
 } mpic_batch_stats;
 
 /* Builds the ensemble from host meshes (mpic_mesh_brick), borrowed for the call. */

@@ -8,7 +8,10 @@
 // A step is a fixed sequence of launches, one per phase for all problems and no
 // host synchronisation in between, so it can be captured in a HIP graph:
 //   node charge of the deposit -> clear the deposit -> per-problem solve (one
-//   workgroup per problem) -> per-tet field -> push through the last ranking
+//   workgroup per problem: conjugate gradients preconditioned by a multigrid
+//   V-cycle on each problem's own hierarchy when every problem's brick
+//   coarsens, else FP32 CG inside FP64 refinement) -> per-tet field -> push
+//   through the last ranking
 //   (kick, walk, FP64 location, deposit, counts) -> ranking by element ->
 //   per-problem kinetic energy.
 // Particles are reflected at walls (no kills), so every problem keeps its
@@ -20,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "mg.hpp"
 #include "mpic_ctx.hpp"
 
 using namespace mpic;
@@ -44,6 +48,16 @@ struct mpic_batch {
   double* phi_save = nullptr;               // tuner: potential before the trials
   unsigned long long* tune_acc = nullptr;   // tuner: scratch deposit
   int32_t* tune_count = nullptr;
+  // the multigrid solve: per-problem hierarchies in pooled device arrays
+  bool mg_on = false;
+  std::string mg_why;                       // why the refinement solve runs instead
+  BatchMGProb* d_mg_prob = nullptr;
+  MGLevelDev* d_mg_lev = nullptr;
+  float *d_mg_dinv0 = nullptr, *d_mg_cinv = nullptr;
+  double* d_mg_val0_64 = nullptr;
+  float* d_mg_val0_32 = nullptr;
+  int mg_max_slots = 0;
+  double *mg_p = nullptr, *mg_q = nullptr;
   int push_grid = 0, solve_threads = 256;
   bool tuned = false, filled = false;
   bool graph_on = false;
@@ -150,6 +164,144 @@ PushArgs32 push_args(mpic_batch* b, double dt, unsigned long long* acc, int32_t*
   return a;
 }
 
+BatchMGArgs mg_args(mpic_batch* b) {
+  mpic_ctx* c = b->c;
+  BatchMGArgs a;
+  a.m = diag_first(c->m, c->m32);
+  a.val32 = c->m32.ell_val;
+  a.prob = b->d_mg_prob;
+  a.lev = b->d_mg_lev;
+  a.dinv0 = b->d_mg_dinv0;
+  a.cinv = b->d_mg_cinv;
+  a.val0_64 = b->d_mg_val0_64;
+  a.val0_32 = b->d_mg_val0_32;
+  a.x = c->phi;
+  a.b = c->rhs;
+  a.r = b->r64;
+  a.p = b->mg_p;
+  a.q = b->mg_q;
+  a.b0 = b->rf;
+  a.x0 = b->wf;
+  a.t0 = b->pf;
+  a.iters = b->d_iters;
+  a.converged = b->d_conv;
+  a.relres = b->d_relres;
+  a.tol2 = c->cfg.cg_tol * c->cfg.cg_tol;
+  a.max_iter = c->cfg.cg_max_iter;
+  return a;
+}
+
+// Every problem's solve with `threads` per workgroup.
+hipError_t launch_solve(mpic_batch* b, int threads, hipStream_t s) {
+  return b->mg_on ? launch_batch_mg(mg_args(b), b->nprob, threads, b->mg_max_slots, s)
+                  : launch_batch_ir(solve_args(b), b->nprob, threads, s);
+}
+
+// The multigrid hierarchy of every problem (mg_build on its brick, the
+// coarsest level at most BATCH_MG_DIRECT_MAX unknowns) in pooled device
+// arrays.  b->mg_on stays false, with the reason in b->mg_why, when multigrid
+// is switched off or some problem's mesh does not coarsen: then every problem
+// runs the refinement solve.
+int build_mg(mpic_batch* b, const std::vector<mpic_mesh_desc>& d, const mpic_config& cc) {
+  b->mg_on = false;
+  if (!cc.multigrid) {
+    b->mg_why = "multigrid switched off (cfg.multigrid = 0)";
+    return MPIC_OK;
+  }
+  const int P = b->nprob;
+  std::vector<MGHost> hs(P);
+  for (int p = 0; p < P; ++p) {
+    const mpic_mesh_desc& m = d[p];
+    std::string why;
+    if (m.lattice[0] <= 0 || m.lattice[1] <= 0 || m.lattice[2] <= 0) {
+      b->mg_why = "problem " + std::to_string(p) + ": its mesh descriptor names no lattice";
+      return MPIC_OK;
+    }
+    if (!mg_build(m.lattice[0], m.lattice[1], m.lattice[2], m.num_nodes, m.boundary, m.row_ptr, m.col_idx, m.values,
+                  hs[p], why, BATCH_MG_DIRECT_MAX)) {
+      b->mg_why = "problem " + std::to_string(p) + ": " + why;
+      return MPIC_OK;
+    }
+  }
+  int64_t npts = 0, ninv = 0, nv0 = 0;
+  int nlev = 0;
+  b->mg_max_slots = 0;
+  for (const MGHost& h : hs) {
+    for (const MGHostLevel& L : h.levels) npts += L.n;
+    nlev += (int)h.levels.size();
+    ninv += (int64_t)h.coarse_inv.size();
+    nv0 += (int64_t)h.val0.size();
+    b->mg_max_slots = std::max(b->mg_max_slots, (int)h.slot0.size());
+  }
+  const int32_t nn = b->c->m.nn;
+  float *val, *dinv, *vb, *vx, *vt;
+  int rc;
+  if ((rc = balloc(b, &val, (size_t)27 * npts)) || (rc = balloc(b, &dinv, (size_t)npts)) ||
+      (rc = balloc(b, &vb, (size_t)npts)) || (rc = balloc(b, &vx, (size_t)npts)) || (rc = balloc(b, &vt, (size_t)npts)) ||
+      (rc = balloc(b, &b->d_mg_cinv, (size_t)ninv)) || (rc = balloc(b, &b->d_mg_dinv0, (size_t)nn)) ||
+      (rc = balloc(b, &b->mg_p, (size_t)nn)) || (rc = balloc(b, &b->mg_q, (size_t)nn)) ||
+      (rc = balloc(b, &b->d_mg_lev, (size_t)nlev)) || (rc = balloc(b, &b->d_mg_prob, (size_t)P)) ||
+      (rc = balloc(b, &b->d_mg_val0_64, (size_t)nv0)) || (rc = balloc(b, &b->d_mg_val0_32, (size_t)nv0)))
+    return rc;
+  std::vector<float> hval((size_t)27 * npts), hdinv(npts), hcinv(ninv), hdinv0(nn, 1.0f), hv32(nv0);
+  std::vector<double> hv64(nv0);
+  int64_t vo = 0;
+  std::vector<MGLevelDev> hlev(nlev);
+  std::vector<BatchMGProb> hprob(P);
+  int64_t po = 0, co = 0;
+  int lo = 0;
+  for (int p = 0; p < P; ++p) {
+    const MGHost& h = hs[p];
+    BatchMGProb& Q = hprob[p];
+    Q.nx = h.nx;
+    Q.ny = h.ny;
+    Q.nz = h.nz;
+    Q.r0 = b->node_off[p];
+    Q.nn = b->node_off[p + 1] - b->node_off[p];
+    Q.lev0 = lo;
+    Q.nlev = (int)h.levels.size();
+    Q.cinv_off = co;
+    Q.v0_off = vo;
+    Q.s0 = (int)h.slot0.size();
+    for (int k = 0; k < 27; ++k) {
+      const int sl = k < Q.s0 ? h.slot0[k] : 13;
+      Q.off0[k] = ((sl / 9 - 1) * (h.ny + 1) + (sl / 3) % 3 - 1) * (h.nz + 1) + sl % 3 - 1;
+    }
+    std::copy(h.val0.begin(), h.val0.end(), hv64.begin() + vo);
+    for (size_t k = 0; k < h.val0.size(); ++k) hv32[vo + k] = (float)h.val0[k];
+    vo += (int64_t)h.val0.size();
+    std::copy(h.dinv0.begin(), h.dinv0.end(), hdinv0.begin() + Q.r0);
+    for (const MGHostLevel& L : h.levels) {
+      MGLevelDev& D = hlev[lo++];
+      D.cx = L.cx;
+      D.cy = L.cy;
+      D.cz = L.cz;
+      D.n = L.n;
+      D.val = val + 27 * po;
+      D.dinv = dinv + po;
+      D.b = vb + po;
+      D.x = vx + po;
+      D.t = vt + po;
+      std::copy(L.val.begin(), L.val.end(), hval.begin() + 27 * po);
+      std::copy(L.dinv.begin(), L.dinv.end(), hdinv.begin() + po);
+      po += L.n;
+    }
+    std::copy(h.coarse_inv.begin(), h.coarse_inv.end(), hcinv.begin() + co);
+    co += (int64_t)h.coarse_inv.size();
+  }
+  BCHK(b, hipMemcpy(val, hval.data(), hval.size() * 4, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(dinv, hdinv.data(), hdinv.size() * 4, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_cinv, hcinv.data(), hcinv.size() * 4, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_dinv0, hdinv0.data(), hdinv0.size() * 4, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_lev, hlev.data(), hlev.size() * sizeof(MGLevelDev), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_prob, hprob.data(), hprob.size() * sizeof(BatchMGProb), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_val0_64, hv64.data(), hv64.size() * 8, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_mg_val0_32, hv32.data(), hv32.size() * 4, hipMemcpyHostToDevice));
+  b->mg_on = true;
+  b->mg_why.clear();
+  return MPIC_OK;
+}
+
 int push_grid_used(mpic_batch* b) {
   return b->push_grid > 0 ? std::min(b->push_grid, PUSH32_MAX_GRID) : push32_grid(b->c->n);
 }
@@ -163,7 +315,7 @@ int enqueue_step(mpic_batch* b, double dt) {
   const int32_t ne = c->m.ne;
   BCHK(b, launch_node_charge64(c->m, c->acc64, c->S64, c->bg64, c->rhs, s));
   BCHK(b, launch_zero(c->acc64, (size_t)ne * 8, s));
-  BCHK(b, launch_batch_ir(solve_args(b), b->nprob, b->solve_threads, s));
+  BCHK(b, launch_solve(b, b->solve_threads, s));
   BCHK(b, launch_field32(c->m, c->phi, c->m32.E, s));
   BCHK(b, launch_zero(c->sb.count, (size_t)(((int64_t)c->m.nbuckets + 2) / 2) * 8, s));
   BCHK(b, launch_push32(push_args(b, dt, c->acc64, c->sb.count, b->push_grid), s));
@@ -303,6 +455,11 @@ int mpic_batch_create(int nprob, mpic_mesh* const* meshes, const mpic_config* cf
     mpic_batch_destroy(b);
     return MPIC_ERR_HIP;
   }
+  if ((rc = build_mg(b, d, cc))) {
+    global_error() = b->err;
+    mpic_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return MPIC_OK;
 }
@@ -433,7 +590,7 @@ int mpic_batch_tune(mpic_batch* b, double dt) {
   for (int k = 0; k < 3; ++k) {
     BCHK(b, hipMemcpyAsync(c->phi, b->phi_save, (size_t)nn * 8, hipMemcpyDeviceToDevice, s));
     BCHK(b, hipEventRecord(b->ev0, s));
-    BCHK(b, launch_batch_ir(solve_args(b), b->nprob, cand_t[k], s));
+    BCHK(b, launch_solve(b, cand_t[k], s));
     BCHK(b, hipEventRecord(b->ev1, s));
     BCHK(b, hipEventSynchronize(b->ev1));
     float ms = 0;
@@ -547,6 +704,7 @@ int mpic_batch_step(mpic_batch* b, double dt, mpic_batch_stats* st) {
     st->capped = fin.capped;
     st->nonpositive = fin.nonpos;
     st->ke_rescaled = rescale ? 1 : 0;
+    st->multigrid = b->mg_on ? 1 : 0;
   }
   if (fin.capped > 0) return bfail(b, MPIC_ERR_WALK_CAP, "a particle exceeded the crossing cap");
   return MPIC_OK;

@@ -499,15 +499,29 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     std::vector<int32_t> pos(adj_ptr.begin(), adj_ptr.end() - 1);
     for (int64_t s = 0; s < (int64_t)ne * 4; ++s) adj[pos[row_of[s]]++] = (int32_t)s;
   }
-  // Sliced ELL from CSR.
+  // Sliced ELL from CSR, each row's entries in their CSR order without the
+  // stored exact zeros (the Kuhn split of a uniform brick stores 15 entries a
+  // row of which 8 cancel to 0.0 exactly).  A zero entry adds a signed zero to
+  // a row sum, which changes nothing but the sign of an exactly zero sum, so
+  // every row sum is the CSR's.  The diagonal is always kept.
   const int32_t nsl = (nn + ELL_SLICE - 1) / ELL_SLICE;
+  std::vector<int32_t> keep_ptr(nn + 1, 0), keep;
+  keep.reserve(d->nnz);
+  for (int32_t r = 0; r < nn; ++r) {
+    for (int32_t e = d->row_ptr[r]; e < d->row_ptr[r + 1]; ++e) {
+      const int32_t cc = d->col_idx[e];
+      if (cc < 0 || cc >= nn) return bail(fail(c, MPIC_ERR_ARG, "mpic_create: CSR column out of range"));
+      if (d->values[e] != 0.0 || cc == r) keep.push_back(e);
+    }
+    keep_ptr[r + 1] = (int32_t)keep.size();
+  }
   std::vector<int64_t> ell_off(nsl + 1, 0);
   std::vector<int32_t> ell_len(nsl, 0);
   for (int32_t s = 0; s < nsl; ++s) {
     int32_t L = 0;
     for (int l = 0; l < ELL_SLICE; ++l) {
       const int64_t r = (int64_t)s * ELL_SLICE + l;
-      if (r < nn) L = std::max(L, d->row_ptr[r + 1] - d->row_ptr[r]);
+      if (r < nn) L = std::max(L, keep_ptr[r + 1] - keep_ptr[r]);
     }
     ell_len[s] = L;
     ell_off[s + 1] = ell_off[s] + (int64_t)L * ELL_SLICE;
@@ -520,11 +534,10 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
       const int64_t r = (int64_t)s * ELL_SLICE + l;
       for (int k = 0; k < ell_len[s]; ++k) {
         const int64_t at = ell_off[s] + (int64_t)k * ELL_SLICE + l;
-        if (r < nn && k < d->row_ptr[r + 1] - d->row_ptr[r]) {
-          const int32_t cc = d->col_idx[d->row_ptr[r] + k];
-          if (cc < 0 || cc >= nn) return bail(fail(c, MPIC_ERR_ARG, "mpic_create: CSR column out of range"));
-          ell_col[at] = cc;
-          ell_val[at] = d->values[d->row_ptr[r] + k];
+        if (r < nn && k < keep_ptr[r + 1] - keep_ptr[r]) {
+          const int32_t e = keep[keep_ptr[r] + k];
+          ell_col[at] = d->col_idx[e];
+          ell_val[at] = d->values[e];
         } else {
           ell_col[at] = r < nn ? (int32_t)r : 0;
           ell_val[at] = 0.0;

@@ -23,92 +23,7 @@
 namespace mpic {
 namespace {
 
-template <int T>
-__device__ __forceinline__ void tree2(double& a, double& b, double* ws) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    a = a + __shfl_xor(a, off);
-    b = b + __shfl_xor(b, off);
-  }
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    ws[threadIdx.x >> 6] = a;
-    ws[T / 64 + (threadIdx.x >> 6)] = b;
-  }
-  __syncthreads();
-  double sa = ws[0], sb = ws[T / 64];
-#pragma unroll
-  for (int w = 1; w < T / 64; ++w) {
-    sa = sa + ws[w];
-    sb = sb + ws[T / 64 + w];
-  }
-  a = sa;
-  b = sb;
-}
-
-// Rows take their first BROW entries in one batch (column indices, then values
-// and the gathered vector entries, all loads in flight before the sum) and any
-// further entries in a loop.  The sum runs over the row's entries in stored
-// order, one fma per entry, as a plain loop over the row would.
-constexpr int BROW = 15;
 constexpr int UROWS = 4;  // rows of a thread updated together
-
-__device__ __forceinline__ double row64(const DevMesh& m, int row, const double* __restrict__ x) {
-  const int slice = row >> 6, lane = row & 63;
-  const int len = m.ell_len[slice];
-  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
-  const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
-  int c[BROW];
-#pragma unroll
-  for (int k = 0; k < BROW; ++k) c[k] = k < len ? cp[k * 64] : row;
-  double acc = 0;
-#pragma unroll
-  for (int k = 0; k < BROW; ++k) {
-    const double v = k < len ? vp[k * 64] : 0.0;
-    const double xv = x[c[k]];
-    if (k < len) acc += v * xv;
-  }
-  for (int k = BROW; k < len; ++k) acc += vp[k * 64] * x[cp[k * 64]];
-  return acc;
-}
-
-// Row sums of NR rows at once, x indexed by global node number (it may point
-// into LDS, offset by the problem's first node, which the compiler resolves
-// after inlining).  Rows with ok[r] false take no entries (their w is 0);
-// their row[r] must still be a row of the mesh.
-template <int NR>
-__device__ __forceinline__ void rows32(const DevMesh& m, const float* __restrict__ val, const int (&row)[NR],
-                                       const bool (&ok)[NR], const float* __restrict__ x, float (&w)[NR]) {
-  int len[NR];
-  int64_t off[NR];
-  int c[NR][BROW];
-  float v[NR][BROW];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int slice = row[r] >> 6;
-    len[r] = ok[r] ? m.ell_len[slice] : 0;
-    off[r] = m.ell_off[slice] + (row[r] & 63);
-  }
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int k = 0; k < BROW; ++k) c[r][k] = k < len[r] ? m.ell_col[off[r] + k * 64] : row[r];
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int k = 0; k < BROW; ++k) v[r][k] = k < len[r] ? val[off[r] + k * 64] : 0.0f;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    float acc = 0;
-#pragma unroll
-    for (int k = 0; k < BROW; ++k) {
-      const float xv = x[c[r][k]];
-      if (k < len[r]) acc = fmaf(v[r][k], xv, acc);
-    }
-    for (int k = BROW; k < len[r]; ++k) acc = fmaf(val[off[r] + k * 64], x[m.ell_col[off[r] + k * 64]], acc);
-    w[r] = acc;
-  }
-}
 
 // LN > 0: the inner residual (the only vector the row sums gather) lives in
 // LDS, LN entries, and every problem has at most LN nodes; LN == 0: in a.rf.
@@ -140,7 +55,7 @@ __global__ __launch_bounds__(T) void k_batch_ir(BatchSolveArgs a) {
         ok[r] = i + r * T < r1;
         row[r] = ok[r] ? i + r * T : i;
       }
-      rows32<SROWS>(m, a.val32, row, ok, rv, w);
+      ell_rows32<SROWS>(m, a.val32, row, ok, rv, w);
 #pragma unroll
       for (int r = 0; r < SROWS; ++r)
         if (ok[r]) {
@@ -159,12 +74,12 @@ __global__ __launch_bounds__(T) void k_batch_ir(BatchSolveArgs a) {
     double tb = 0, tr = 0;
     for (int i = r0 + threadIdx.x; i < r1; i += T) {
       const double bi = a.b[i];
-      const double ri = bi - row64(m, i, a.x);
+      const double ri = bi - ell_row64(m, i, a.x);
       a.r64[i] = ri;
       tb += bi * bi;
       tr += ri * ri;
     }
-    tree2<T>(tb, tr, ws);
+    wg_sum2<T>(tb, tr, ws);
     bb = tb;
     rr = tr;
     conv = rr == 0 || rr < a.tol2 * bb;
@@ -182,7 +97,7 @@ __global__ __launch_bounds__(T) void k_batch_ir(BatchSolveArgs a) {
     __syncthreads();
     double g = 0, d = 0;
     spmv(g, d);
-    tree2<T>(g, d, ws);
+    wg_sum2<T>(g, d, ws);
     double rho0 = g, rho_prev = 1.0, alpha_prev = 1.0;
     int it = 0;
     for (;; ++it) {
@@ -229,7 +144,7 @@ __global__ __launch_bounds__(T) void k_batch_ir(BatchSolveArgs a) {
       __syncthreads();
       // w = A r; r.r and w.r
       spmv(g, d);
-      tree2<T>(g, d, ws);
+      wg_sum2<T>(g, d, ws);
     }
     inner_total += it;
     // x += |r| d (FP64)

@@ -26,8 +26,9 @@
 // once.  Dot products use the fixed chunk tree of the other solves (chunk =
 // 256 rows, partials reduced redundantly by every workgroup of the next
 // launch), so results do not depend on scheduling.
-#include "mpic_internal.hpp"
+#include "mg_device.hpp"
 #include "mpic_device.hpp"
+#include "mpic_internal.hpp"
 
 namespace mpic {
 namespace {
@@ -61,68 +62,9 @@ __device__ __forceinline__ bool wg_done(const MGState* st) {
   return flag != 0;
 }
 
-__device__ __forceinline__ float w1(int d) { return d == 0 ? 1.0f : 0.5f; }
-
-// Coarse level points: interior lattice points (1..c-1 per direction).
-__device__ __forceinline__ int cidx(const MGLevelDev& L, int i, int j, int k) {
-  return ((i - 1) * (L.cy - 1) + (j - 1)) * (L.cz - 1) + (k - 1);
-}
-__device__ __forceinline__ void cdecode(const MGLevelDev& L, int u, int& i, int& j, int& k) {
-  const int ez = L.cz - 1, ey = L.cy - 1;
-  k = u % ez + 1;
-  j = (u / ez) % ey + 1;
-  i = u / (ez * ey) + 1;
-}
-__device__ __forceinline__ bool cinterior(const MGLevelDev& L, int i, int j, int k) {
-  return i > 0 && i < L.cx && j > 0 && j < L.cy && k > 0 && k < L.cz;
-}
 // Level 0 points: every node, numbered as the brick generator does.
 __device__ __forceinline__ int nidx(const MGBuffers& g, int i, int j, int k) {
   return (i * (g.ny + 1) + j) * (g.nz + 1) + k;
-}
-
-// Level 0 row sum over the FP32 sliced ELL (diagonal first), four entries per batch.
-__device__ __forceinline__ float row32(const DevMesh& m, const float* __restrict__ val, int row,
-                                       const float* __restrict__ v) {
-  const int slice = row >> 6, lane = row & 63;
-  const int len = m.ell_len[slice];
-  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
-  const float* __restrict__ vp = val + m.ell_off[slice] + lane;
-  float acc = 0.0f;
-  int k = 0;
-  for (; k + 4 <= len; k += 4) {
-    const int c0 = cp[(k + 0) * 64], c1 = cp[(k + 1) * 64], c2 = cp[(k + 2) * 64], c3 = cp[(k + 3) * 64];
-    const float a0 = vp[(k + 0) * 64], a1 = vp[(k + 1) * 64], a2 = vp[(k + 2) * 64], a3 = vp[(k + 3) * 64];
-    const float x0 = v[c0], x1 = v[c1], x2 = v[c2], x3 = v[c3];
-    acc += a0 * x0;
-    acc += a1 * x1;
-    acc += a2 * x2;
-    acc += a3 * x3;
-  }
-  for (; k < len; ++k) acc += vp[k * 64] * v[cp[k * 64]];
-  return acc;
-}
-
-// Level 0 FP64 row sum with the operand formed at each column by f(col).
-template <class F>
-__device__ __forceinline__ double row64(const DevMesh& m, int row, F f) {
-  const int slice = row >> 6, lane = row & 63;
-  const int len = m.ell_len[slice];
-  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
-  const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
-  double acc = 0.0;
-  int k = 0;
-  for (; k + 4 <= len; k += 4) {
-    const int c0 = cp[(k + 0) * 64], c1 = cp[(k + 1) * 64], c2 = cp[(k + 2) * 64], c3 = cp[(k + 3) * 64];
-    const double a0 = vp[(k + 0) * 64], a1 = vp[(k + 1) * 64], a2 = vp[(k + 2) * 64], a3 = vp[(k + 3) * 64];
-    const double x0 = f(c0), x1 = f(c1), x2 = f(c2), x3 = f(c3);
-    acc += a0 * x0;
-    acc += a1 * x1;
-    acc += a2 * x2;
-    acc += a3 * x3;
-  }
-  for (; k < len; ++k) acc += vp[k * 64] * f(cp[k * 64]);
-  return acc;
 }
 
 struct MGArgs {
@@ -137,51 +79,16 @@ struct MGArgs {
 
 // ---- restriction, prolongation and the tiles of the coarse levels ----
 
-// P^T t of the finer level at point (I, J, K) of a coarse level: full
-// weighting of the fine points 2I - 1..2I + 1 (per direction) that are
-// interior: 2I + 1 is the fine wall when the fine cell count is odd, and takes
-// weight 0 (its load, at the clamped point 2I, stays in bounds; all 27 loads
-// are issued unconditionally).  The finer level is level 0 (t0 on the node
-// lattice) when F.n == 0, else F.
+// P^T t of the finer level at point (I, J, K) of a coarse level: level 0 (t0
+// on the node lattice) when F.n == 0, else F.
 __device__ __forceinline__ float restrict_at(const MGBuffers& g, const MGLevelDev& F, int I, int J, int K) {
-  float s = 0.0f;
-  const bool top = F.n == 0;
-  const bool okx = 2 * I + 1 < (top ? g.nx : F.cx), oky = 2 * J + 1 < (top ? g.ny : F.cy),
-             okz = 2 * K + 1 < (top ? g.nz : F.cz);
-  const float hx = okx ? 0.5f : 0.0f, hy = oky ? 0.5f : 0.0f, hz = okz ? 0.5f : 0.0f;
-  const int ux = okx ? 1 : 0, uy = oky ? 1 : 0, uz = okz ? 1 : 0;
-  const float wx[3] = {0.5f, 1.0f, hx}, wy[3] = {0.5f, 1.0f, hy}, wz[3] = {0.5f, 1.0f, hz};
-  const int ox[3] = {-1, 0, ux}, oy[3] = {-1, 0, uy}, oz[3] = {-1, 0, uz};
-  if (top) {
-    const float* __restrict__ t = g.t0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          s += (wx[a] * wy[b] * wz[c]) * t[nidx(g, 2 * I + ox[a], 2 * J + oy[b], 2 * K + oz[c])];
-  } else {
-    const float* __restrict__ t = F.t;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          s += (wx[a] * wy[b] * wz[c]) * t[cidx(F, 2 * I + ox[a], 2 * J + oy[b], 2 * K + oz[c])];
-  }
-  return s;
+  if (F.n == 0) return mg_restrict(g.t0, [&](int i, int j, int k) { return nidx(g, i, j, k); }, g.nx, g.ny, g.nz, I, J, K);
+  return mg_restrict(F.t, [&](int i, int j, int k) { return mg_cidx(F, i, j, k); }, F.cx, F.cy, F.cz, I, J, K);
 }
 
 // P z of level lc + 1 at point (i, j, k) of level lc (z is in that level's b).
 __device__ __forceinline__ float prolong_at(const MGLevelDev& C, int i, int j, int k) {
-  float s = 0.0f;
-  for (int I = i >> 1; I <= (i + 1) >> 1; ++I)
-    for (int J = j >> 1; J <= (j + 1) >> 1; ++J)
-      for (int K = k >> 1; K <= (k + 1) >> 1; ++K)
-        if (cinterior(C, I, J, K)) s += (w1(i - 2 * I) * w1(j - 2 * J) * w1(k - 2 * K)) * C.b[cidx(C, I, J, K)];
-  return s;
+  return mg_prolong(C, i, j, k);
 }
 
 // A coarse level in tiles of TX x TY x TZ points, one per workgroup (one point
@@ -275,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) void k_mg_init(MGArgs a) {
   if (i < a.m.nn) {
     const double* __restrict__ x = a.x;
     const double bi = a.b[i];
-    const double ri = bi - row64(a.m, i, [&](int c) { return x[c]; });
+    const double ri = bi - mg_row64(a.m, i, [&](int c) { return x[c]; });
     a.g.r[i] = ri;
     const float bf = (float)ri;
     a.g.b0[i] = bf;
@@ -313,7 +220,7 @@ __global__ __launch_bounds__(BLOCK) void k_mg_resid0(MGArgs a) {
     }
   }
   if (stop) return;
-  if (i < a.m.nn) a.g.t0[i] = a.g.b0[i] - row32(a.m, a.val32, i, a.g.x0);
+  if (i < a.m.nn) a.g.t0[i] = a.g.b0[i] - mg_row32(a.m, a.val32, i, a.g.x0);
 }
 
 // The coarse-level kernels take their levels by value (the descriptors'
@@ -331,16 +238,16 @@ __global__ __launch_bounds__(BLOCK) void k_mg_rr(MGArgs a, MGLevelDev L, MGLevel
     int i, j, k;
     halo_point(T, h, i, j, k);
     float b = 0.0f, x = 0.0f;
-    if (cinterior(L, i, j, k)) {
+    if (mg_cinterior(L, i, j, k)) {
       b = restrict_at(a.g, F, i, j, k);
-      x = MG_OMEGA * L.dinv[cidx(L, i, j, k)] * b;
+      x = MG_OMEGA * L.dinv[mg_cidx(L, i, j, k)] * b;
     }
     bs[h] = b;
     xs[h] = x;
   }
   __syncthreads();
-  if (!cinterior(L, T.i, T.j, T.k)) return;
-  const int u = cidx(L, T.i, T.j, T.k);
+  if (!mg_cinterior(L, T.i, T.j, T.k)) return;
+  const int u = mg_cidx(L, T.i, T.j, T.k);
   float acc = 0.0f;
 #pragma unroll
   for (int e = 0; e < 27; ++e) acc += L.val[e * L.n + u] * xs[T.hc + halo_off(e)];
@@ -358,13 +265,13 @@ __global__ __launch_bounds__(BLOCK) void k_mg_pp(MGArgs a, MGLevelDev L, MGLevel
   __shared__ float ys[HALO];
   if (a.g.st->done) return;
   const TileAt T = tile_at(L);
-  const bool own = cinterior(L, T.i, T.j, T.k);
-  const int u = own ? cidx(L, T.i, T.j, T.k) : 0;
+  const bool own = mg_cinterior(L, T.i, T.j, T.k);
+  const int u = own ? mg_cidx(L, T.i, T.j, T.k) : 0;
   const float bu = own ? L.b[u] : 0.0f, du = own ? L.dinv[u] : 0.0f;
   for (int h = threadIdx.x; h < HALO; h += BLOCK) {
     int i, j, k;
     halo_point(T, h, i, j, k);
-    ys[h] = cinterior(L, i, j, k) ? L.x[cidx(L, i, j, k)] + prolong_at(C, i, j, k) : 0.0f;
+    ys[h] = mg_cinterior(L, i, j, k) ? L.x[mg_cidx(L, i, j, k)] + prolong_at(C, i, j, k) : 0.0f;
   }
   __syncthreads();
   if (!own) return;
@@ -398,7 +305,7 @@ __global__ __launch_bounds__(BLOCK) void k_mg_cs(MGArgs a, MGLevelDev C, MGLevel
     float b = 0.0f;  // zero past n: the clamped reads below multiply it by zero
     if (u < n) {
       int I, J, K;
-      cdecode(C, u, I, J, K);
+      mg_cdecode(C, u, I, J, K);
       b = restrict_at(a.g, F, I, J, K);
     }
     bsh[u] = b;
@@ -449,7 +356,7 @@ __global__ __launch_bounds__(BLOCK) void k_mg_post0(MGArgs a) {
   const int i = chunk * BLOCK + threadIdx.x;
   double t = 0.0;
   if (i < a.m.nn) {
-    const float z = a.g.t0[i] + MG_OMEGA * a.g.dinv0[i] * (a.g.b0[i] - row32(a.m, a.val32, i, a.g.t0));
+    const float z = a.g.t0[i] + MG_OMEGA * a.g.dinv0[i] * (a.g.b0[i] - mg_row32(a.m, a.val32, i, a.g.t0));
     a.g.b0[i] = z;
     t = a.g.r[i] * (double)z;
   }
@@ -476,9 +383,9 @@ __global__ __launch_bounds__(BLOCK) void k_mg_ap(MGArgs a) {
   if (i < a.m.nn) {
     double qi;
     if (fresh)
-      qi = row64(a.m, i, [&](int c) { return (double)z[c]; });
+      qi = mg_row64(a.m, i, [&](int c) { return (double)z[c]; });
     else
-      qi = row64(a.m, i, [&](int c) { return (double)z[c] + beta * po[c]; });
+      qi = mg_row64(a.m, i, [&](int c) { return (double)z[c] + beta * po[c]; });
     const double pi = fresh ? (double)z[i] : (double)z[i] + beta * po[i];
     pn[i] = pi;
     a.g.q[i] = qi;

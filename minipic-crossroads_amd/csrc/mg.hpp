@@ -43,15 +43,23 @@ struct MGHostLevel {
 struct MGHost {
   int nx = 0, ny = 0, nz = 0;         // level 0 cells (the mesh)
   std::vector<float> dinv0;           // [nn] level 0 diagonal inverse
+  // Level 0's matrix as a stencil over all nodes: slot s couples node n to
+  // node n + (di (ny + 1) + dj) (nz + 1) + dk, where slot0[s] = (di + 1) 9 +
+  // (dj + 1) 3 + (dk + 1); the slots any row uses, in ascending order (so a
+  // row's entries run in ascending column order); values [S][nn], zero where
+  // the row has no such entry (wall rows keep only their diagonal).
+  std::vector<int> slot0;
+  std::vector<double> val0;
   std::vector<MGHostLevel> levels;    // levels 1..L; the last is solved directly
   std::vector<float> coarse_inv;      // inverse of the coarsest operator, column-major ([j][i] = (i, j))
 };
 
 // Builds the hierarchy from the FP64 stiffness matrix in CSR (rows of the
 // brick's nodes, numbered (i (ny + 1) + j) (nz + 1) + k; Dirichlet rows are
-// identity rows).  false (with the reason) when multigrid does not apply.
+// identity rows), coarsening until a level has at most direct_max unknowns
+// (<= MG_DIRECT_MAX).  false (with the reason) when multigrid does not apply.
 bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const int32_t* row_ptr, const int32_t* col,
-              const double* val, MGHost& out, std::string& why);
+              const double* val, MGHost& out, std::string& why, int direct_max = MG_DIRECT_MAX);
 
 }  // namespace mpic
 

@@ -144,8 +144,12 @@ bool spd_inverse(int n, std::vector<double>& a) {
 }  // namespace
 
 bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const int32_t* row_ptr, const int32_t* col,
-              const double* val, MGHost& out, std::string& why) {
+              const double* val, MGHost& out, std::string& why, int direct_max) {
   out = MGHost{};
+  if (direct_max < 1 || direct_max > MG_DIRECT_MAX) {
+    why = "bad size of the directly solved level";
+    return false;
+  }
   if (nx < 3 || ny < 3 || nz < 3 || (int64_t)(nx + 1) * (ny + 1) * (nz + 1) != nn) {
     why = "the mesh is not a brick lattice of at least 3 cells per direction";
     return false;
@@ -210,10 +214,10 @@ bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const
     out.levels.push_back(std::move(L));
     gf = gc;
     af.swap(ac);
-    if (gf.count() <= MG_DIRECT_MAX) break;
+    if (gf.count() <= direct_max) break;
   }
   const int64_t nc = gf.count();
-  if (out.levels.empty() || nc > MG_DIRECT_MAX) {
+  if (out.levels.empty() || nc > direct_max) {
     why = "the coarsest level has " + std::to_string(nc) +
           " unknowns (a direction with fewer than 3 cells stops the coarsening)";
     return false;
@@ -243,6 +247,28 @@ bool mg_build(int nx, int ny, int nz, int32_t nn, const uint8_t* boundary, const
   for (int32_t r = 0; r < nn; ++r)
     for (int32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
       if (col[e] == r && val[e] != 0.0) out.dinv0[r] = (float)(1.0 / val[e]);
+  // level 0 as a stencil over all nodes (interior rows were checked to couple
+  // neighbours at most one cell apart; wall rows are identity rows)
+  {
+    const int ny1 = ny + 1, nz1 = nz + 1;
+    bool used[27] = {false};
+    auto slot_of = [&](int32_t r, int32_t c) {
+      const int ri = r / (ny1 * nz1), rj = (r / nz1) % ny1, rk = r % nz1;
+      const int ci = c / (ny1 * nz1), cj = (c / nz1) % ny1, ck = c % nz1;
+      return (ci - ri + 1) * 9 + (cj - rj + 1) * 3 + (ck - rk + 1);
+    };
+    for (int32_t r = 0; r < nn; ++r)
+      for (int32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+        if (val[e] != 0.0) used[slot_of(r, col[e])] = true;
+    for (int s = 0; s < 27; ++s)
+      if (used[s]) out.slot0.push_back(s);
+    int idx[27];
+    for (size_t k = 0; k < out.slot0.size(); ++k) idx[out.slot0[k]] = (int)k;
+    out.val0.assign(out.slot0.size() * (size_t)nn, 0.0);
+    for (int32_t r = 0; r < nn; ++r)
+      for (int32_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+        if (val[e] != 0.0) out.val0[(size_t)idx[slot_of(r, col[e])] * nn + r] += val[e];
+  }
   return true;
 }
 

@@ -18,6 +18,132 @@ __device__ __forceinline__ int xcd_chunk(int b, int n) {
   return c < n ? c : -1;
 }
 
+// Two sums over a workgroup of T threads: xor butterflies in each wave, then the
+// wave totals added in wave order by every thread (the same order everywhere,
+// so every thread holds the same sums).  Starts and ends with barriers around
+// its LDS words ws[2 T / 64].
+template <int T>
+__device__ __forceinline__ void wg_sum2(double& a, double& b, double* ws) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    a = a + __shfl_xor(a, off);
+    b = b + __shfl_xor(b, off);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    ws[threadIdx.x >> 6] = a;
+    ws[T / 64 + (threadIdx.x >> 6)] = b;
+  }
+  __syncthreads();
+  double sa = ws[0], sb = ws[T / 64];
+#pragma unroll
+  for (int w = 1; w < T / 64; ++w) {
+    sa = sa + ws[w];
+    sb = sb + ws[T / 64 + w];
+  }
+  a = sa;
+  b = sb;
+}
+
+// Rows take their first ELL_BATCH entries in one batch (column indices, then values
+// and the gathered vector entries, all loads in flight before the sum) and any
+// further entries in a loop.  The sum runs over the row's entries in stored
+// order, one fma per entry, as a plain loop over the row would.
+constexpr int ELL_BATCH = 15;
+
+__device__ __forceinline__ double ell_row64(const DevMesh& m, int row, const double* __restrict__ x) {
+  const int slice = row >> 6, lane = row & 63;
+  const int len = m.ell_len[slice];
+  const int* __restrict__ cp = m.ell_col + m.ell_off[slice] + lane;
+  const double* __restrict__ vp = m.ell_val + m.ell_off[slice] + lane;
+  int c[ELL_BATCH];
+#pragma unroll
+  for (int k = 0; k < ELL_BATCH; ++k) c[k] = k < len ? cp[k * 64] : row;
+  double acc = 0;
+#pragma unroll
+  for (int k = 0; k < ELL_BATCH; ++k) {
+    const double v = k < len ? vp[k * 64] : 0.0;
+    const double xv = x[c[k]];
+    if (k < len) acc += v * xv;
+  }
+  for (int k = ELL_BATCH; k < len; ++k) acc += vp[k * 64] * x[cp[k * 64]];
+  return acc;
+}
+
+// Row sums of NR rows at once, x indexed by global node number (it may point
+// into LDS, offset by the problem's first node, which the compiler resolves
+// after inlining).  Rows with ok[r] false take no entries (their w is 0);
+// their row[r] must still be a row of the mesh.
+template <int NR>
+__device__ __forceinline__ void ell_rows32(const DevMesh& m, const float* __restrict__ val, const int (&row)[NR],
+                                           const bool (&ok)[NR], const float* __restrict__ x, float (&w)[NR]) {
+  int len[NR];
+  int64_t off[NR];
+  int c[NR][ELL_BATCH];
+  float v[NR][ELL_BATCH];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int slice = row[r] >> 6;
+    len[r] = ok[r] ? m.ell_len[slice] : 0;
+    off[r] = m.ell_off[slice] + (row[r] & 63);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) c[r][k] = k < len[r] ? m.ell_col[off[r] + k * 64] : row[r];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) v[r][k] = k < len[r] ? val[off[r] + k * 64] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    float acc = 0;
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) {
+      const float xv = x[c[r][k]];
+      if (k < len[r]) acc = fmaf(v[r][k], xv, acc);
+    }
+    for (int k = ELL_BATCH; k < len[r]; ++k) acc = fmaf(val[off[r] + k * 64], x[m.ell_col[off[r] + k * 64]], acc);
+    w[r] = acc;
+  }
+}
+
+// FP64 row sums of NR rows at once, the operand of column c given by f(c);
+// the same batching and entry order as ell_rows32.
+template <int NR, class F>
+__device__ __forceinline__ void ell_rows64(const DevMesh& m, const int (&row)[NR], const bool (&ok)[NR], F f,
+                                           double (&w)[NR]) {
+  int len[NR];
+  int64_t off[NR];
+  int c[NR][ELL_BATCH];
+  double v[NR][ELL_BATCH];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int slice = row[r] >> 6;
+    len[r] = ok[r] ? m.ell_len[slice] : 0;
+    off[r] = m.ell_off[slice] + (row[r] & 63);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) c[r][k] = k < len[r] ? m.ell_col[off[r] + k * 64] : row[r];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) v[r][k] = k < len[r] ? m.ell_val[off[r] + k * 64] : 0.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < ELL_BATCH; ++k) {
+      const double xv = f(c[r][k]);
+      if (k < len[r]) acc += v[r][k] * xv;
+    }
+    for (int k = ELL_BATCH; k < len[r]; ++k) acc += m.ell_val[off[r] + k * 64] * f(m.ell_col[off[r] + k * 64]);
+    w[r] = acc;
+  }
+}
+
 // A thread's share of P chunk partials, part[t] + part[t + BLOCK] + ... in that
 // order (t = threadIdx.x), with the first PART_BATCH loads issued together: a
 // plain loop waits for each load before the next (one memory round trip per

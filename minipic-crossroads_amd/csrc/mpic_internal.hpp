@@ -493,6 +493,42 @@ struct BatchSolveArgs {
   int max_iter;
   int max_nodes;            // nodes of the largest problem
 };
+// The batching form's multigrid solve (k_batch_mg.hip): conjugate gradients in
+// FP64 preconditioned by an FP32 V-cycle, one workgroup per problem, each
+// problem with its own hierarchy (mg_build on its brick, coarsest level at most
+// BATCH_MG_DIRECT_MAX unknowns) in pooled device arrays.
+constexpr int BATCH_MG_DIRECT_MAX = 128;
+struct BatchMGProb {
+  int nx, ny, nz;    // cells of the problem's brick (level 0)
+  int r0, nn;        // its node range in the union
+  int lev0, nlev;    // its coarse levels 1..nlev are lev[lev0 .. lev0 + nlev - 1]
+  int64_t cinv_off;  // inverse of its coarsest operator at cinv + cinv_off (column-major)
+  int64_t v0_off;    // its level-0 stencil values at val0_* + v0_off ([s0][nn])
+  int s0;            // stencil slots of level 0
+  int off0[27];      // node offset of each slot's column
+};
+struct BatchMGArgs {
+  DevMesh m;                 // diagonal-first ELL, FP64 values
+  const float* val32;        // FP32 values of the same ELL
+  const BatchMGProb* prob;   // [nprob]
+  const MGLevelDev* lev;     // the coarse levels of every problem
+  const float* dinv0;        // [nn] level-0 diagonal inverse of every problem
+  const float* cinv;         // the coarsest inverses
+  const double* val0_64;     // level-0 stencil values of every problem, FP64
+  const float* val0_32;      // the same in FP32
+  double* x;                 // potential (warm start)
+  const double* b;           // right-hand side
+  double *r, *p, *q;         // FP64 CG vectors [nn]
+  float *b0, *x0, *t0;       // level-0 V-cycle vectors [nn]; b0 holds z after a cycle
+  int* iters;                // [nprob] iterations of the last solve
+  int* converged;            // [nprob]
+  double* relres;            // [nprob] |r| / |b| at the end
+  double tol2;
+  int max_iter;
+};
+// Every problem's multigrid solve, one workgroup of `threads` (256, 512 or 1024) per problem;
+// max_slots: the largest level-0 stencil (<= 27).
+hipError_t launch_batch_mg(const BatchMGArgs& a, int nprob, int threads, int max_slots, hipStream_t s);
 // LDS capacities (nodes) of the batch solve's inner residual: problems up to
 // BATCH_LDS_SMALL nodes (64 KB) or BATCH_LDS_LARGE (144 KB); larger ones keep
 // it in device memory.

@@ -13,7 +13,9 @@
 //      |r| < 1e-10 |b| from zero in few iterations, independent of the mesh
 //      size, where plain CG needs hundreds;
 //   3. a direction of fewer than 3 cells and a periodic mesh are refused with
-//      a reason.
+//      a reason;
+//   4. level 0 as a stencil (slots of the lattice offsets) gives the CSR's
+//      row sums bit for bit (same entries, ascending columns).
 // Prints one line per case and "mg_check ok", exit 0; or the failure, exit 1.
 #include <cmath>
 #include <cstdio>
@@ -279,6 +281,26 @@ void pcg_case(const Case& cs, int max_pcg_its) {
       const std::vector<float> z = V.run(0, rf);
       return std::vector<double>(z.begin(), z.end());
     };
+    // 4. the level-0 stencil against the CSR rows, every node
+    {
+      std::vector<double> xr(A.n), y;
+      for (int i = 0; i < A.n; ++i) xr[i] = nd(gen);
+      spmv(A, xr, y);
+      const int S = (int)h.slot0.size(), ny1 = cs.ny + 1, nz1 = cs.nz + 1;
+      long bad = 0;
+      for (int r = 0; r < A.n; ++r) {
+        double acc = 0;
+        for (int s = 0; s < S; ++s) {
+          const int sl = h.slot0[s];
+          const int c = r + ((sl / 9 - 1) * ny1 + (sl / 3) % 3 - 1) * nz1 + sl % 3 - 1;
+          const double v = h.val0[(size_t)s * A.n + r];
+          if (v != 0.0) acc += v * xr[c];
+        }
+        bad += acc != y[r];
+      }
+      std::printf("stencil %dx%dx%d: %d slots, %ld rows differ from the CSR\n", cs.nx, cs.ny, cs.nz, S, bad);
+      check(bad == 0 && S <= 27, "level-0 stencil");
+    }
     const int it_mg = pcg(A, b, M, 1e-10, 200);
     const int it_cg = pcg(A, b, [](const std::vector<double>& r) { return r; }, 1e-10, 5000);
     std::printf("pcg %dx%dx%d split %d mapping %d jitter %.2f: %zu levels, multigrid %d iterations, plain CG %d\n",

@@ -25,6 +25,11 @@ def say(msg):
 SIDES = [3, 5, 4, 6, 2]
 NPPE = [4, 10, 7, 12, 16]
 VTH = [0.05, 0.2, 0.1, 0.15, 0.08]
+# every side >= 3: each problem solves by CG preconditioned by a multigrid
+# V-cycle on its own hierarchy (odd and even cell counts); SIDES has a
+# 2-cell box, which does not coarsen, so its ensemble runs the refinement solve
+MG_SIDES = [3, 5, 4, 6, 7]
+SIDE_SETS = pytest.mark.parametrize("sides,mg", [(SIDES, 0), (MG_SIDES, 1)], ids=["refinement", "multigrid"])
 
 
 def fills(drift=0.0):
@@ -32,8 +37,8 @@ def fills(drift=0.0):
             for k in range(len(SIDES))]
 
 
-def make_batch(mp, **cfg):
-    meshes = [mp.Mesh(n, n, n, 6) for n in SIDES]
+def make_batch(mp, sides=SIDES, **cfg):
+    meshes = [mp.Mesh(n, n, n, 6) for n in sides]
     b = mp.Batch(meshes, mp.default_config(**cfg))
     b.fill(fills())
     return b
@@ -44,12 +49,13 @@ def by_id(g):
     return {k: (v[:, o] if v.ndim == 2 else v[o]) for k, v in g.items()}
 
 
-def test_batch_fill_and_charge_match_single_problems(mp, oracle):
-    say("batch fill and charge vs single problems and the oracle")
-    b = make_batch(mp)
-    b.step(0.05)
+@SIDE_SETS
+def test_batch_fill_and_charge_match_single_problems(mp, oracle, sides, mg):
+    say("batch fill and charge vs single problems and the oracle, sides %s" % sides)
+    b = make_batch(mp, sides)
+    assert b.step(0.05)["multigrid"] == mg
     st = b.problem_stats()
-    for k, n in enumerate(SIDES):
+    for k, n in enumerate(sides):
         pm = mp.Mesh(n, n, n, 6, background=DENSITY)
         ctx = mp.Context(pm.desc, mp.default_config(form=mp.FORM_MIXED))
         ctx.reserve(pm.ne * NPPE[k])
@@ -78,13 +84,18 @@ def test_batch_fill_and_charge_match_single_problems(mp, oracle):
     b.close()
 
 
-def test_batch_steps_match_single_problems(mp):
-    say("batch steps vs single problems (precision form)")
-    b = make_batch(mp)
+@SIDE_SETS
+def test_batch_steps_match_single_problems(mp, sides, mg):
+    say("batch steps vs single problems (precision form), sides %s" % sides)
+    b = make_batch(mp, sides)
     for _ in range(3):
-        b.step(0.05)
+        sb = b.step(0.05)
+        assert sb["multigrid"] == mg and sb["all_converged"] == 1
+        if mg:
+            assert 0 < sb["max_inner_iters"] <= 30, sb["max_inner_iters"]
     st = b.problem_stats()
-    for k, n in enumerate(SIDES):
+    assert np.all(st["relres"] < 1e-10)
+    for k, n in enumerate(sides):
         pm = mp.Mesh(n, n, n, 6, background=DENSITY)
         ctx = mp.Context(pm.desc, mp.default_config(form=mp.FORM_MIXED))
         ctx.reserve(pm.ne * NPPE[k])
@@ -106,19 +117,21 @@ def test_batch_steps_match_single_problems(mp):
 
 
 def test_batch_solve_storage_paths(mp):
-    """The batch solve keeps the inner residual in LDS up to 16384 nodes (64 KB),
-    in a larger LDS tile up to 36864 nodes, in device memory beyond: sides 26
-    (19683 nodes) and 34 (42875 nodes) take the last two, each with a small
-    problem beside it, and match the same problems run alone."""
-    for sides in ((26, 3), (34, 2)):
-        say("batch solve, sides %s" % (sides,))
+    """The refinement solve of the batch (multigrid switched off) keeps the
+    inner residual in LDS up to 16384 nodes (64 KB), in a larger LDS tile up to
+    36864 nodes, in device memory beyond: sides 26 (19683 nodes) and 34 (42875
+    nodes) take the last two, each with a small problem beside it, and match
+    the same problems run alone.  The multigrid solve of the same ensembles
+    matches them too."""
+    for sides, mgc in (((26, 3), 0), ((34, 2), 0), ((26, 3), 1)):
+        say("batch solve, sides %s, multigrid %d" % (sides, mgc))
         nppe = [2, 5]
         meshes = [mp.Mesh(n, n, n, 6) for n in sides]
-        b = mp.Batch(meshes, mp.default_config())
+        b = mp.Batch(meshes, mp.default_config(multigrid=mgc))
         b.fill([dict(nppe=nppe[k], temp=2 * 0.1 ** 2, density=DENSITY, pert_amp=0.05, seed_offset=k + 1)
                 for k in range(2)])
         st = b.step(0.05)
-        assert st["all_converged"] == 1
+        assert st["all_converged"] == 1 and st["multigrid"] == mgc
         ps = b.problem_stats()
         for k, n in enumerate(sides):
             assert ps["count"][k] == n ** 3 * 6 * nppe[k]
@@ -134,20 +147,21 @@ def test_batch_solve_storage_paths(mp):
         b.close()
 
 
-def test_batch_graph_replays_equal_eager(mp):
-    say("batch graph replays vs eager steps")
-    ea = make_batch(mp)
-    gr = make_batch(mp)
+@SIDE_SETS
+def test_batch_graph_replays_equal_eager(mp, sides, mg):
+    say("batch graph replays vs eager steps, sides %s" % sides)
+    ea = make_batch(mp, sides)
+    gr = make_batch(mp, sides)
     gr.graph(True)
     for s in range(5):
         se = ea.step(0.05)
         sg = gr.step(0.05)  # first two steps capture (one graph per buffer parity), then replays
-        assert sg["graph"] == 1
+        assert sg["graph"] == 1 and sg["multigrid"] == se["multigrid"] == mg
         pe, pg = ea.problem_stats(), gr.problem_stats()
-        for k in range(len(SIDES)):
+        for k in range(len(sides)):
             assert np.array_equal(ea.phi(k), gr.phi(k)), (s, k)
         assert np.array_equal(pe["ke"], pg["ke"]) and np.array_equal(pe["iters"], pg["iters"]), s
-    for k in range(len(SIDES)):
+    for k in range(len(sides)):
         n = int(ea.problem_stats()["count"][k])
         a, g = by_id(ea.download(k, n)), by_id(gr.download(k, n))
         for key in ("x", "v", "elem"):
@@ -155,22 +169,23 @@ def test_batch_graph_replays_equal_eager(mp):
     # after a download (which reorders the list) the graph path continues correctly
     ea.step(0.05)
     gr.step(0.05)
-    for k in range(len(SIDES)):
+    for k in range(len(sides)):
         assert np.array_equal(ea.phi(k), gr.phi(k)), k
     ea.close()
     gr.close()
 
 
-def test_batch_tune(mp):
-    say("batch autotuner")
-    b = make_batch(mp)
-    phi0 = [b.phi(k) for k in range(len(SIDES))]
+@SIDE_SETS
+def test_batch_tune(mp, sides, mg):
+    say("batch autotuner, sides %s" % sides)
+    b = make_batch(mp, sides)
+    phi0 = [b.phi(k) for k in range(len(sides))]
     b.tune(0.05)
-    for k in range(len(SIDES)):
+    for k in range(len(sides)):
         assert np.array_equal(b.phi(k), phi0[k])  # tuning advances nothing
     st = b.step(0.05)
     assert st["tuned"] == 1 and st["push_grid"] in (2048, 4096, 8192, 16384)
-    assert st["solve_threads"] in (256, 512, 1024) and st["all_converged"] == 1
+    assert st["solve_threads"] in (256, 512, 1024) and st["all_converged"] == 1 and st["multigrid"] == mg
     assert b.check_ownership() == 0
     b.close()
 
@@ -212,7 +227,7 @@ def test_batch_ensemble_c4(mp):
     b.graph(True)
     for s in range(50):
         st = b.step(0.05)
-        assert st["all_converged"] == 1, s
+        assert st["all_converged"] == 1 and st["multigrid"] == 1, s
         if s % 10 == 0:
             say("  step %d: %.2f ms, max inner iterations %d" % (s, st["ms_step"], st["max_inner_iters"]))
     ps = b.problem_stats()
