@@ -48,14 +48,38 @@ __device__ __forceinline__ float mg_restrict(const float* __restrict__ t, Idx id
   return s;
 }
 
-// P z of the coarser level C at point (i, j, k) of the finer one (z is in C.b).
+// P z of the coarser level C at point (i, j, k) of the finer one (z is in C.b):
+// per direction the parents i >> 1 (weight 1 for even i, 1/2 for odd) and
+// (i >> 1) + 1 (weight 1/2 for odd i, 0 for even); a parent on the wall takes
+// weight 0 and its load goes to a clamped interior point, so all 8 loads are
+// issued unconditionally (the zero-weight terms add nothing to the sum).
 __device__ __forceinline__ float mg_prolong(const MGLevelDev& C, int i, int j, int k) {
+  int I[2], J[2], K[2];
+  float wi[2], wj[2], wk[2];
+  auto parents = [](int f, int c, int(&P)[2], float(&w)[2]) {
+    const int p0 = f >> 1;
+    const bool odd = f & 1;
+    w[0] = odd ? 0.5f : 1.0f;
+    w[1] = odd ? 0.5f : 0.0f;
+    P[0] = p0;
+    P[1] = p0 + 1;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (P[q] < 1 || P[q] > c - 1) {
+        w[q] = 0.0f;
+        P[q] = min(max(P[q], 1), c - 1);
+      }
+  };
+  parents(i, C.cx, I, wi);
+  parents(j, C.cy, J, wj);
+  parents(k, C.cz, K, wk);
   float s = 0.0f;
-  for (int I = i >> 1; I <= (i + 1) >> 1; ++I)
-    for (int J = j >> 1; J <= (j + 1) >> 1; ++J)
-      for (int K = k >> 1; K <= (k + 1) >> 1; ++K)
-        if (mg_cinterior(C, I, J, K))
-          s += (mg_w1(i - 2 * I) * mg_w1(j - 2 * J) * mg_w1(k - 2 * K)) * C.b[mg_cidx(C, I, J, K)];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) s += (wi[a] * wj[b] * wk[c]) * C.b[mg_cidx(C, I[a], J[b], K[c])];
   return s;
 }
 
