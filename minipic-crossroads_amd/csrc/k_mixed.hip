@@ -213,28 +213,33 @@ __device__ __forceinline__ bool walk32(const DevMesh& m, const float* __restrict
 }
 
 // ---- element charge table (LDS), 64-bit fixed point --------------------------
-constexpr int DEP64 = 1024;  // entries (power of two)
-constexpr int DEP64_BITS = 10;
-
+// N entries (a power of two) keyed by element, linear probing; a key that finds
+// neither itself nor a free entry within 64 probes goes straight to the global
+// accumulator (the table is flushed long before it fills).
+template <int N>
 struct DepTable64 {
-  int key[DEP64];
-  int cnt[DEP64];
-  unsigned long long val[DEP64];
+  static_assert((N & (N - 1)) == 0, "power of two");
+  int key[N];
+  int cnt[N];
+  unsigned long long val[N];
 };
 
-__device__ __forceinline__ void dep64_clear(DepTable64& t) {
-  for (int k = threadIdx.x; k < DEP64; k += BLOCK) {
+template <int N>
+__device__ __forceinline__ void dep64_clear(DepTable64<N>& t) {
+  for (int k = threadIdx.x; k < N; k += BLOCK) {
     t.key[k] = DEP_EMPTY;
     t.cnt[k] = 0;
     t.val[k] = 0;
   }
 }
 
-__device__ __forceinline__ void dep64_add(DepTable64& t, unsigned long long* __restrict__ acc, int* __restrict__ count,
-                                          const int32_t* __restrict__ ckey, int key, long long c) {
+template <int N>
+__device__ __forceinline__ void dep64_add(DepTable64<N>& t, unsigned long long* __restrict__ acc,
+                                          int* __restrict__ count, const int32_t* __restrict__ ckey, int key,
+                                          long long c) {
   if (key < 0) return;
-  unsigned h = ((unsigned)key * 2654435761u) >> (32 - DEP64_BITS);
-  for (int probe = 0; probe < DEP64; ++probe) {
+  unsigned h = ((unsigned)key * 2654435761u) >> (32 - __builtin_ctz(N));
+  for (int probe = 0; probe < (N < 64 ? N : 64); ++probe) {
     int k = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == DEP_EMPTY) {
       int expected = DEP_EMPTY;
@@ -247,16 +252,17 @@ __device__ __forceinline__ void dep64_add(DepTable64& t, unsigned long long* __r
       atomicAdd(&t.cnt[h], 1);
       return;
     }
-    h = (h + 1) & (DEP64 - 1);
+    h = (h + 1) & (N - 1);
   }
   __hip_atomic_fetch_add(acc + key, (unsigned long long)c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (count) atomicAdd(count + ckey[key], 1);
 }
 
-__device__ __forceinline__ void dep64_flush(DepTable64& t, unsigned long long* __restrict__ acc, int* __restrict__ count,
-                                            const int32_t* __restrict__ ckey) {
+template <int N>
+__device__ __forceinline__ void dep64_flush(DepTable64<N>& t, unsigned long long* __restrict__ acc,
+                                            int* __restrict__ count, const int32_t* __restrict__ ckey) {
   __syncthreads();
-  for (int k = threadIdx.x; k < DEP64; k += BLOCK) {
+  for (int k = threadIdx.x; k < N; k += BLOCK) {
     const int key = t.key[k];
     if (key != DEP_EMPTY) {
       const unsigned long long v = t.val[k];
@@ -267,6 +273,88 @@ __device__ __forceinline__ void dep64_flush(DepTable64& t, unsigned long long* _
   __syncthreads();
   dep64_clear(t);
   __syncthreads();
+}
+constexpr int DEP64 = 1024;       // k_deposit32: a flush covers FLUSH_CHUNKS * BLOCK = 1024 particles
+constexpr int DEP64_PUSH = 512;   // k_push32 (its LDS also holds the wavefront exchange areas)
+
+// ---- particle records through LDS ---------------------------------------------
+// The vector L1 address path costs about 16 cycles per wavefront load or store
+// of 16 B per lane, plus more for every further 64-byte line that a group of
+// four lanes touches.  A wavefront's 64 consecutive 48-byte records are
+// 3 KB: read or written one record per lane they touch three lines per lane
+// group in each of three instructions; as contiguous 16-byte pieces (lane l
+// takes pieces l, 64 + l, 128 + l) they touch one.  The pieces pass to and
+// from the lanes that own the records through a per-wavefront LDS area.
+constexpr int REC_PIECES = 3 * 64;
+
+// Orders this wavefront's LDS accesses across lanes (a wavefront executes its
+// LDS instructions in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// This lane's index, opaque to the optimiser, so that lane-dependent LDS
+// offsets are computed where they are used instead of occupying registers
+// across the particle loop.
+__device__ __forceinline__ int fresh_lane() {
+  int l = __lane_id();
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
+// Records s0 .. s0 + nv - 1 to lanes 0 .. nv - 1 (others: elem = -1).
+__device__ __forceinline__ Rec32 wave_load_recs(float4* __restrict__ sl, const Rec32* __restrict__ P, int64_t s0,
+                                                int nv) {
+  const int lane = fresh_lane(), n3 = 3 * nv;
+  const float4* in = reinterpret_cast<const float4*>(P + s0);
+  float4 v0, v1, v2;
+  if (lane < n3) v0 = in[lane];
+  if (64 + lane < n3) v1 = in[64 + lane];
+  if (128 + lane < n3) v2 = in[128 + lane];
+  if (lane < n3) sl[lane] = v0;
+  if (64 + lane < n3) sl[64 + lane] = v1;
+  if (128 + lane < n3) sl[128 + lane] = v2;
+  wave_sync();
+  Rec32 r{};
+  r.elem = -1;
+  if (lane < nv) {
+    const float4 a0 = sl[3 * lane], a1 = sl[3 * lane + 1];
+    const int4 a2 = reinterpret_cast<const int4*>(sl)[3 * lane + 2];
+    r.x[0] = a0.x;
+    r.x[1] = a0.y;
+    r.x[2] = a0.z;
+    r.elem = __float_as_int(a0.w);
+    r.v[0] = a1.x;
+    r.v[1] = a1.y;
+    r.v[2] = a1.z;
+    r.type = (uint32_t)__float_as_int(a1.w);
+    r.w = __hiloint2double(a2.y, a2.x);
+    r.id = (uint32_t)a2.z;
+    r.pad = 0;
+  }
+  wave_sync();
+  return r;
+}
+
+// Lane l's record (l < nv) to slot s0 + l: whole records, or only the position
+// and velocity words (the weight/id word unchanged).
+__device__ __forceinline__ void wave_store_recs(float4* __restrict__ sl, Rec32* __restrict__ Q, int64_t s0, int nv,
+                                                bool whole, const Rec32& r) {
+  const int lane = fresh_lane(), n3 = 3 * nv;
+  if (lane < nv) {
+    sl[3 * lane] = float4{r.x[0], r.x[1], r.x[2], __int_as_float(r.elem)};
+    sl[3 * lane + 1] = float4{r.v[0], r.v[1], r.v[2], __int_as_float((int)r.type)};
+    reinterpret_cast<int4*>(sl)[3 * lane + 2] = int4{__double2loint(r.w), __double2hiint(r.w), (int)r.id, 0};
+  }
+  wave_sync();
+  float4* out = reinterpret_cast<float4*>(Q + s0);
+  // piece 64 k + lane is part (lane + k) mod 3 of its record; part 2 is the weight/id word
+  const int part = lane % 3;
+  if (lane < n3 && (whole || part != 2)) out[lane] = sl[lane];
+  if (64 + lane < n3 && (whole || part != 1)) out[64 + lane] = sl[64 + lane];
+  if (128 + lane < n3 && (whole || part != 0)) out[128 + lane] = sl[128 + lane];
+  wave_sync();
 }
 
 // ---- kernels -----------------------------------------------------------------
@@ -289,9 +377,13 @@ __global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __re
 }
 
 __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
-  __shared__ DepTable64 table;
+  __shared__ DepTable64<DEP64_PUSH> table;
+  __shared__ float4 recx[BLOCK / 64][REC_PIECES];
   __shared__ RareCount rare;
+  __shared__ double sp[48];
+  float4* sl = recx[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
   dep64_clear(table);
+  if (threadIdx.x < 48) sp[threadIdx.x] = a.species[threadIdx.x];
   if (threadIdx.x == 0) rare = RareCount{0, 0, 0, 0, 0};
   const bool gather = a.perm != nullptr;
   int crossings = 0;
@@ -303,13 +395,18 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
   int chunk = 0;
   for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
     const int64_t p = base + threadIdx.x;  // output position
+    const int64_t s0 = base + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64;  // the wavefront's first
+    const int64_t rem = hi - s0;
+    const int nv = rem <= 0 ? 0 : rem >= 64 ? 64 : (int)rem;
     int enew = -1;
     double w = 0;
     float V[3] = {0, 0, 0};
     int t = 0;
+    Rec32 rc{};
+    rc.elem = -1;
+    if (!gather && nv > 0) rc = wave_load_recs(sl, a.P.r, s0, nv);
     if (p < hi) {
-      const int64_t src = gather ? (int64_t)a.perm[p] : p;
-      Rec32 rc = load_rec(a.P.r, src);
+      if (gather) rc = load_rec(a.P.r, (int64_t)a.perm[p]);
       int e = rc.elem;
       if (e >= 0) {
         float X[3], R[3];
@@ -328,7 +425,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
           p2r_vecf(g.J, dl, R);
         }
         // Boris push with B = 0 (two half kicks), particle_move.hpp:364-392
-        const float hq = a.dt * a.scale * (float)a.species[32 + t] * 0.5f;
+        const float hq = a.dt * a.scale * (float)sp[32 + t] * 0.5f;
         const float Ev[3] = {E.x, E.y, E.z};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -354,24 +451,15 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
         enew = alive ? e : -1;
         if (!alive) atomicAdd(&rare.killed, 1);
         rc.elem = enew;
-        if (gather) {
-          store_rec(a.Q.r, p, rc);
-        } else {
-          // in place: the (weight, id) word does not change
-          float4* q = reinterpret_cast<float4*>(a.Q.r + p);
-          q[0] = float4{rc.x[0], rc.x[1], rc.x[2], __int_as_float(rc.elem)};
-          q[1] = float4{rc.v[0], rc.v[1], rc.v[2], __int_as_float((int)rc.type)};
-        }
-        a.Q.elem[p] = enew;
-      } else if (gather) {
-        store_rec(a.Q.r, p, rc);
-        a.Q.elem[p] = -1;
       }
     }
+    // records back (a dead particle's as read: the ranking drops it), then the element mirror
+    if (nv > 0) wave_store_recs(sl, a.Q.r, s0, nv, gather, rc);
+    if (p < hi) a.Q.elem[p] = rc.elem < 0 ? -1 : rc.elem;
     if (enew >= 0) {
-      dep64_add(table, a.acc, a.count, a.m.bucket, enew, to_fixed64(w * a.species[t], a.S_charge));
+      dep64_add(table, a.acc, a.count, a.m.bucket, enew, to_fixed64(w * sp[t], a.S_charge));
       const double Vd[3] = {(double)V[0], (double)V[1], (double)V[2]};
-      const double term = ke_term(w, a.species[16 + t], Vd);
+      const double term = ke_term(w, sp[16 + t], Vd);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
       const unsigned tb = (unsigned)__double2hiint(term);
       kmax_hi = tb > kmax_hi ? tb : kmax_hi;
@@ -397,7 +485,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
 // Deposit of the current positions (weight_charge, PIC.cpp:139-160).
 __global__ __launch_bounds__(BLOCK) void k_deposit32(DevMesh m, DevParticles32 P, int64_t n, const double* sp, int S,
                                                      unsigned long long* acc) {
-  __shared__ DepTable64 table;
+  __shared__ DepTable64<DEP64> table;
   dep64_clear(table);
   int64_t lo, hi;
   block_range(n, &lo, &hi);
