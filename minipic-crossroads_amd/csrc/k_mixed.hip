@@ -337,6 +337,48 @@ __device__ __forceinline__ Rec32 wave_load_recs(float4* __restrict__ sl, const R
   return r;
 }
 
+// Lanes 0 .. nv - 1 receive records src (their own slot each, on steps that
+// read through the ranking); the wavefront reads them as pieces j = 64 k + lane
+// (part j % 3 of the record of lane j / 3), so that a lane group touches two
+// records instead of four.
+__device__ __forceinline__ Rec32 wave_gather_recs(float4* __restrict__ sl, int* __restrict__ key,
+                                                  const Rec32* __restrict__ P, int src, int nv) {
+  const int lane = fresh_lane(), n3 = 3 * nv;
+  if (lane < nv) key[lane] = src;
+  wave_sync();
+  auto piece = [&](int j) -> float4 {
+    const int r = j / 3;
+    return reinterpret_cast<const float4*>(P + key[r])[j - 3 * r];
+  };
+  float4 v0, v1, v2;
+  if (lane < n3) v0 = piece(lane);
+  if (64 + lane < n3) v1 = piece(64 + lane);
+  if (128 + lane < n3) v2 = piece(128 + lane);
+  if (lane < n3) sl[lane] = v0;
+  if (64 + lane < n3) sl[64 + lane] = v1;
+  if (128 + lane < n3) sl[128 + lane] = v2;
+  wave_sync();
+  Rec32 r{};
+  r.elem = -1;
+  if (lane < nv) {
+    const float4 a0 = sl[3 * lane], a1 = sl[3 * lane + 1];
+    const int4 a2 = reinterpret_cast<const int4*>(sl)[3 * lane + 2];
+    r.x[0] = a0.x;
+    r.x[1] = a0.y;
+    r.x[2] = a0.z;
+    r.elem = __float_as_int(a0.w);
+    r.v[0] = a1.x;
+    r.v[1] = a1.y;
+    r.v[2] = a1.z;
+    r.type = (uint32_t)__float_as_int(a1.w);
+    r.w = __hiloint2double(a2.y, a2.x);
+    r.id = (uint32_t)a2.z;
+    r.pad = 0;
+  }
+  wave_sync();
+  return r;
+}
+
 // Lane l's record (l < nv) to slot s0 + l: whole records, or only the position
 // and velocity words (the weight/id word unchanged).
 __device__ __forceinline__ void wave_store_recs(float4* __restrict__ sl, Rec32* __restrict__ Q, int64_t s0, int nv,
@@ -379,9 +421,11 @@ __global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __re
 __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
   __shared__ DepTable64<DEP64_PUSH> table;
   __shared__ float4 recx[BLOCK / 64][REC_PIECES];
+  __shared__ int recsrc[BLOCK / 64][64];
   __shared__ RareCount rare;
   __shared__ double sp[48];
   float4* sl = recx[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+  int* skey = recsrc[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
   dep64_clear(table);
   if (threadIdx.x < 48) sp[threadIdx.x] = a.species[threadIdx.x];
   if (threadIdx.x == 0) rare = RareCount{0, 0, 0, 0, 0};
@@ -404,9 +448,13 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
     int t = 0;
     Rec32 rc{};
     rc.elem = -1;
-    if (!gather && nv > 0) rc = wave_load_recs(sl, a.P.r, s0, nv);
+    if (nv > 0) {
+      if (gather)
+        rc = wave_gather_recs(sl, skey, a.P.r, p < hi ? a.perm[p] : 0, nv);
+      else
+        rc = wave_load_recs(sl, a.P.r, s0, nv);
+    }
     if (p < hi) {
-      if (gather) rc = load_rec(a.P.r, (int64_t)a.perm[p]);
       int e = rc.elem;
       if (e >= 0) {
         float X[3], R[3];
