@@ -287,22 +287,6 @@ constexpr int DEP64_PUSH = 512;   // k_push32 (its LDS also holds the wavefront 
 // from the lanes that own the records through a per-wavefront LDS area.
 constexpr int REC_PIECES = 3 * 64;
 
-// Orders this wavefront's LDS accesses across lanes (a wavefront executes its
-// LDS instructions in order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-// This lane's index, opaque to the optimiser, so that lane-dependent LDS
-// offsets are computed where they are used instead of occupying registers
-// across the particle loop.
-__device__ __forceinline__ int fresh_lane() {
-  int l = __lane_id();
-  asm volatile("" : "+v"(l));
-  return l;
-}
-
 // Records s0 .. s0 + nv - 1 to lanes 0 .. nv - 1 (others: elem = -1).
 __device__ __forceinline__ Rec32 wave_load_recs(float4* __restrict__ sl, const Rec32* __restrict__ P, int64_t s0,
                                                 int nv) {

@@ -213,6 +213,103 @@ __global__ __launch_bounds__(BLOCK) void k_elem_phi(DevMesh m, const double* __r
 }
 
 
+// ---- FP64 particle records through LDS -------------------------------------------
+// As in the precision form: a wavefront's 64 consecutive 96-byte records (6 KB)
+// are read and written as contiguous 16-byte pieces (lane l takes pieces
+// 64 k + l), one 64-byte line per lane quad instead of six, and pass to and from
+// the lanes that own them through a 3 KB LDS area, half the wavefront's records
+// at a time (pieces 192 h .. 192 h + 191 are records 32 h .. 32 h + 31).
+constexpr int REC64_HALF = 192;  // 16-byte pieces of 32 records
+
+__device__ __forceinline__ Rec64 rec64_from(const float4* sl) {
+  const double2 a0 = reinterpret_cast<const double2*>(sl)[0], a1 = reinterpret_cast<const double2*>(sl)[1],
+                a2 = reinterpret_cast<const double2*>(sl)[2], a3 = reinterpret_cast<const double2*>(sl)[3],
+                a4 = reinterpret_cast<const double2*>(sl)[4];
+  const int4 a5 = reinterpret_cast<const int4*>(sl)[5];
+  Rec64 r;
+  r.x[0] = a0.x;
+  r.x[1] = a0.y;
+  r.x[2] = a1.x;
+  r.xr[0] = a1.y;
+  r.xr[1] = a2.x;
+  r.xr[2] = a2.y;
+  r.v[0] = a3.x;
+  r.v[1] = a3.y;
+  r.v[2] = a4.x;
+  r.w = a4.y;
+  r.elem = a5.x;
+  r.id = (uint32_t)a5.y;
+  r.type = (uint32_t)a5.z;
+  r.pad = 0;
+  return r;
+}
+
+// Lanes 0 .. nv - 1 receive records: slot s0 + lane (in place), or slot src of
+// each lane (through the ranking; key holds them).  Others get elem = -1.
+__device__ __forceinline__ Rec64 wave_load_recs64(float4* __restrict__ sl, int* __restrict__ key,
+                                                  const Rec64* __restrict__ P, bool gather, int64_t s0, int src,
+                                                  int nv) {
+  const int lane = fresh_lane(), n6 = 6 * nv;
+  if (gather) {
+    if (lane < nv) key[lane] = src;
+    wave_sync();
+  }
+  const float4* in = reinterpret_cast<const float4*>(P + s0);
+  auto piece = [&](int j) -> float4 {
+    if (!gather) return in[j];
+    const int r = j / 6;
+    return reinterpret_cast<const float4*>(P + key[r])[j - 6 * r];
+  };
+  Rec64 r{};
+  r.elem = -1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = REC64_HALF * h;
+    if (b < n6) {
+      float4 v0, v1, v2;
+      if (b + lane < n6) v0 = piece(b + lane);
+      if (b + 64 + lane < n6) v1 = piece(b + 64 + lane);
+      if (b + 128 + lane < n6) v2 = piece(b + 128 + lane);
+      if (b + lane < n6) sl[lane] = v0;
+      if (b + 64 + lane < n6) sl[64 + lane] = v1;
+      if (b + 128 + lane < n6) sl[128 + lane] = v2;
+      wave_sync();
+      const int rr = lane - 32 * h;
+      if (rr >= 0 && rr < 32 && lane < nv) r = rec64_from(sl + 6 * rr);
+      wave_sync();
+    }
+  }
+  return r;
+}
+
+// Lane l's record (l < nv) to slot s0 + l.
+__device__ __forceinline__ void wave_store_recs64(float4* __restrict__ sl, Rec64* __restrict__ Q, int64_t s0, int nv,
+                                                  const Rec64& r) {
+  const int lane = fresh_lane(), n6 = 6 * nv;
+  float4* out = reinterpret_cast<float4*>(Q + s0);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = REC64_HALF * h;
+    if (b < n6) {
+      const int rr = lane - 32 * h;
+      if (rr >= 0 && rr < 32 && lane < nv) {
+        double2* q = reinterpret_cast<double2*>(sl + 6 * rr);
+        q[0] = double2{r.x[0], r.x[1]};
+        q[1] = double2{r.x[2], r.xr[0]};
+        q[2] = double2{r.xr[1], r.xr[2]};
+        q[3] = double2{r.v[0], r.v[1]};
+        q[4] = double2{r.v[2], r.w};
+        reinterpret_cast<int4*>(sl + 6 * rr)[5] = int4{r.elem, (int)r.id, (int)r.type, 0};
+      }
+      wave_sync();
+      if (b + lane < n6) out[b + lane] = sl[lane];
+      if (b + 64 + lane < n6) out[b + 64 + lane] = sl[64 + lane];
+      if (b + 128 + lane < n6) out[b + 128 + lane] = sl[128 + lane];
+      wave_sync();
+    }
+  }
+}
+
 struct KArgs {
   DevMesh m;
   DevParticles P;   // input particles
@@ -233,7 +330,14 @@ struct KArgs {
 
 __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   __shared__ DepTable table;
+  __shared__ float4 recx[BLOCK / 64][REC64_HALF];
+  __shared__ int recsrc[BLOCK / 64][64];
+  __shared__ double sp[48];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float4* sl = recx[wave];
+  int* skey = recsrc[wave];
   dep_table_clear(table);
+  if (threadIdx.x < 48) sp[threadIdx.x] = a.sp[threadIdx.x];
   const bool gather = a.perm != nullptr;
   WalkCount wc{};
   long long killed = 0;
@@ -245,13 +349,18 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
   int chunk = 0;
   for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
     const int64_t p = base + threadIdx.x;  // output position
+    const int64_t s0 = base + wave * 64;   // the wavefront's first
+    const int64_t rem = hi - s0;
+    const int nv = rem <= 0 ? 0 : rem >= 64 ? 64 : (int)rem;
     int enew = -1;
     double w = 0, V[3] = {0, 0, 0};
     int t = 0;
+    // records in: sorted order gathers through the permutation
+    Rec64 rc{};
+    rc.elem = -1;
+    if (nv > 0) rc = wave_load_recs64(sl, skey, a.P.r, gather, s0, gather && p < hi ? a.perm[p] : 0, nv);
     if (p < hi) {
       ElemRec g;
-      const int64_t src = gather ? (int64_t)a.perm[p] : p;  // sorted order: gather through the permutation
-      Rec64 rc = load_rec(a.P.r, src);
       int e = rc.elem;
       if (e >= 0) {
         double X[3], R[3];
@@ -267,7 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         double E[3];
         gather_E(a.m, e, R, E);
         double VR[3];
-        kick(g.J, V, E, a.sp[32 + t], a.dt, a.scale, VR);
+        kick(g.J, V, E, sp[32 + t], a.dt, a.scale, VR);
         const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -278,18 +387,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         enew = alive ? e : -1;
         killed += alive ? 0 : 1;
         rc.elem = enew;
-        store_rec(a.Q.r, p, rc);
-        a.Q.elem[p] = enew;
-      } else if (gather) {
-        store_rec(a.Q.r, p, rc);
-        a.Q.elem[p] = -1;
       }
     }
+    // records out (a dead particle's as read: the ranking drops it), then the element mirror
+    if (nv > 0) wave_store_recs64(sl, a.Q.r, s0, nv, rc);
+    if (p < hi) a.Q.elem[p] = rc.elem < 0 ? -1 : rc.elem;
     if (enew >= 0) {
       fx128 c[4];
-      particle_charge(a.m, enew, a.m.sixodet[enew], w, a.sp[t], a.S_charge, c);
+      particle_charge(a.m, enew, a.m.sixodet[enew], w, sp[t], a.S_charge, c);
       dep_table_add(table, a.acc, a.count, a.m.bucket, enew, c);
-      const double term = ke_term(w, a.sp[16 + t], V);
+      const double term = ke_term(w, sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
       const unsigned long long tb = __double_as_longlong(term);
       kmax = tb > kmax ? tb : kmax;

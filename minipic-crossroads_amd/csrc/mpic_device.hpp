@@ -213,6 +213,23 @@ __device__ __forceinline__ void load16(const double* __restrict__ base, int e, d
   }
 }
 
+// ---- wavefront helpers for records exchanged through LDS ----
+// Orders this wavefront's LDS accesses across lanes (a wavefront executes its
+// LDS instructions in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// This lane's index, opaque to the optimiser, so that lane-dependent LDS
+// offsets are computed where they are used instead of occupying registers
+// across the particle loop.
+__device__ __forceinline__ int fresh_lane() {
+  int l = __lane_id();
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
 // Particle records as 16-B words.
 __device__ __forceinline__ Rec64 load_rec(const Rec64* __restrict__ base, int64_t i) {
   const double2* q = reinterpret_cast<const double2*>(base + i);
