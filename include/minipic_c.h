@@ -82,7 +82,8 @@ typedef struct {
   int cg_max_iter;                       /* CG_solve.hpp:73 uses 1000 */
   int kill_on_wall;                      /* DataWarehouse::set_kill_particles_on_wall_impact, data_warehouse.hpp:76 */
   int sort_period;                       /* reorder the particle list every N time steps (0 = never) */
-  int sort_bucket;                       /* elements per spatial bucket of that reordering (<= 1: by element) */
+  int sort_bucket;                       /* elements per spatial bucket of that reordering (<= 1: by element;
+                                            default 6, about one hex cell of the brick splits) */
   int num_species;                       /* ParticleTypeList, particle_type.hpp:58-86 */
   double species_q[MPIC_MAX_SPECIES];
   double species_m[MPIC_MAX_SPECIES];

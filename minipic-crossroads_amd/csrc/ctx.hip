@@ -394,7 +394,7 @@ void mpic_config_default(mpic_config* cfg) {
   cfg->cg_max_iter = 1000;
   cfg->kill_on_wall = 0;
   cfg->sort_period = 1;
-  cfg->sort_bucket = 1;
+  cfg->sort_bucket = 6;  // about one hex cell of the brick splits (5 or 6 tets)
   cfg->persistent_solve = 0;
   cfg->multigrid = 1;
   cfg->num_species = 3;
