@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "mpic_ctx.hpp"
+#include "p1_cubature.hpp"
 
 namespace mpic {
 std::string& global_error() {
@@ -464,6 +465,23 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     std::memcpy(&erec[e * 16 + 12], &nbrs[e * 4], 4 * sizeof(int32_t));
     erec[e * 16 + 14] = sixodet[e];
   }
+  // The weighted basis values of the configurations' meshes are (detJ w) N_i(x_c), each one IEEE product; when
+  // every value of this mesh is, bit for bit, the FP64 push forms them from detJ w in the walk record instead of
+  // reading 16 doubles per particle (the charge is the same: PIC.cpp:125-130 multiplies by each value).
+  bool factored = ne > 0;
+  for (int64_t e = 0; e < ne && factored; ++e) {
+    const double meas = d->det_jacobian[e] * kCubW;
+    for (int i = 0; i < 4 && factored; ++i)
+      for (int q = 0; q < 4; ++q) {
+        const double v = meas * cub_basis(i, q);
+        if (std::memcmp(&v, d->wbv + e * 16 + i * 4 + q, sizeof(double)) != 0) {
+          factored = false;
+          break;
+        }
+      }
+    erec[e * 16 + 15] = meas;
+  }
+  c->m.wbv_factored = factored ? 1 : 0;
   // Node -> element slot adjacency (slot = e*4 + k), element order.  The charge
   // of a periodic image node goes to its partner's row (PIC.cpp:132-134): the
   // partner gathers the image's slots, the image none.

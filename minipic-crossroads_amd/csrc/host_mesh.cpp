@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "minipic_c.h"
+#include "p1_cubature.hpp"
 
 namespace mpic {
 std::string& global_error();
@@ -45,10 +46,10 @@ constexpr int kKuhn[6][4] = {{0, 4, 6, 7}, {0, 5, 4, 7}, {0, 6, 2, 7}, {0, 2, 3,
 constexpr int kFaces[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
 // Face normal f = (n[C]-n[B]) x (n[A]-n[B]) (mesh.cpp:654-666).
 constexpr int kNa[4] = {0, 1, 0, 0}, kNb[4] = {1, 2, 3, 2}, kNc[4] = {3, 3, 2, 1};
-// P1 gradients on the reference tet and the degree-2, 4-point cubature rule.
+// P1 gradients on the reference tet (the cubature rule: p1_cubature.hpp).
 constexpr double kGradRef[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-constexpr double kCubA = 0.1381966011250105, kCubB = 0.5854101966249685, kCubW = 1.0 / 24.0;
-constexpr double kCub[4][3] = {{kCubA, kCubA, kCubA}, {kCubB, kCubA, kCubA}, {kCubA, kCubB, kCubA}, {kCubA, kCubA, kCubB}};
+using mpic::kCub;
+using mpic::kCubW;
 
 // Mesh::apply_mapping, mesh.cpp:763-792: smooth maps of the brick applied to
 // the node coordinates once topology and boundary sets exist (mesh.cpp:587-588).

@@ -10,6 +10,7 @@
 // without FMA contraction, so results are bit-identical to the CPU restatement.
 #include "mpic_internal.hpp"
 #include "mpic_device.hpp"
+#include "p1_cubature.hpp"
 
 namespace mpic {
 namespace {
@@ -22,6 +23,7 @@ struct ElemRec {
   double n0[3];
   int nb[4];
   double sixodet;
+  double meas;  // detJ / 24 (DevMesh::wbv_factored)
 };
 
 __device__ __forceinline__ ElemRec load_erec(const double* __restrict__ erec, int e) {
@@ -46,6 +48,7 @@ __device__ __forceinline__ ElemRec load_erec(const double* __restrict__ erec, in
   r.nb[2] = a6.z;
   r.nb[3] = a6.w;
   r.sixodet = a7.x;
+  r.meas = a7.y;
   return r;
 }
 
@@ -202,6 +205,22 @@ __device__ __forceinline__ void particle_charge(const DevMesh& m, int e, double 
   }
 }
 
+// The same charge with the weighted basis values formed from the element's
+// detJ/24 (meas) and the cubature rule's basis values: wbv[i][c] = meas * N_i(x_c)
+// bit for bit where DevMesh::wbv_factored says so, so the 16 products and their
+// sums are the reference's.
+__device__ __forceinline__ void particle_charge_factored(double sixodet, double meas, double w, double q, int S,
+                                                         fx128 c[4]) {
+  const double tq = w * q * sixodet;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double acc = 0;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) acc += -tq * (meas * cub_basis(i, cc));
+    c[i] = to_fixed(acc, S);
+  }
+}
+
 // ephi[e][k] = phi[conn[e][k]] (after every solve).
 __global__ __launch_bounds__(BLOCK) void k_elem_phi(DevMesh m, const double* __restrict__ phi) {
   const int e = blockIdx.x * BLOCK + threadIdx.x;
@@ -328,7 +347,7 @@ struct KArgs {
   minipic::ParticleListView view;  // k_move: wall kills go through add_particles_to_migrate_list
 };
 
-__global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
+__global__ __launch_bounds__(BLOCK, 3) void k_push_fused(KArgs a) {
   __shared__ DepTable table;
   __shared__ float4 recx[BLOCK / 64][REC64_HALF];
   __shared__ int recsrc[BLOCK / 64][64];
@@ -354,6 +373,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     const int nv = rem <= 0 ? 0 : rem >= 64 ? 64 : (int)rem;
     int enew = -1;
     double w = 0, V[3] = {0, 0, 0};
+    double sixodet = 0, meas = 0;  // of the final element (its walk record)
     int t = 0;
     // records in: sorted order gathers through the permutation
     Rec64 rc{};
@@ -387,6 +407,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
         enew = alive ? e : -1;
         killed += alive ? 0 : 1;
         rc.elem = enew;
+        sixodet = g.sixodet;
+        meas = g.meas;
       }
     }
     // records out (a dead particle's as read: the ranking drops it), then the element mirror
@@ -394,7 +416,10 @@ __global__ __launch_bounds__(BLOCK) void k_push_fused(KArgs a) {
     if (p < hi) a.Q.elem[p] = rc.elem < 0 ? -1 : rc.elem;
     if (enew >= 0) {
       fx128 c[4];
-      particle_charge(a.m, enew, a.m.sixodet[enew], w, sp[t], a.S_charge, c);
+      if (a.m.wbv_factored)
+        particle_charge_factored(sixodet, meas, w, sp[t], a.S_charge, c);
+      else
+        particle_charge(a.m, enew, a.m.sixodet[enew], w, sp[t], a.S_charge, c);
       dep_table_add(table, a.acc, a.count, a.m.bucket, enew, c);
       const double term = ke_term(w, sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));

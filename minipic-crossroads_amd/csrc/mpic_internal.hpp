@@ -41,7 +41,7 @@ struct DevMesh {
   int32_t nn, ne;
   const double* geo;       // [ne][12]  inverse Jacobian (row-major 3x3) then node-0 coordinates
   // [ne][16]  the walk's element record, one 128-B line: inverse Jacobian (9),
-  // node 0 (3), face neighbours (4 x int32 in 2 slots), 6/detJ, unused
+  // node 0 (3), face neighbours (4 x int32 in 2 slots), 6/detJ, detJ/24 (see wbv_factored)
   const double* erec;
   double* ephi;            // [ne][4]   potential at the element's nodes (refreshed after each solve)
   const double* grad;      // [ne][12]  physical P1 gradients G[e][i][d]
@@ -68,6 +68,9 @@ struct DevMesh {
   const double* ell_val;
   double lo[3], hi[3];
   double period_x;  // lo[0] + hi[0]: x -> period_x - x across a periodic face (neighbour stored as -2 - e)
+  // 1 when every weighted basis value is wbv[e][i][c] = (detJ_e * w) * N_i(x_c) bit for bit (the rule of
+  // p1_cubature.hpp); erec[e][15] then holds detJ_e * w, and the FP64 push forms the 16 values itself
+  int wbv_factored;
 };
 
 // Particles on the device are records (one 16-B aligned record per particle,

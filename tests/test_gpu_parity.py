@@ -253,3 +253,39 @@ def test_full_size_c2_properties(mp, oracle):
         assert np.array_equal(g2[k][:, sel], getattr(Q, k)), k
     assert np.array_equal(g2["elem"][sel], Q.elem)
     ctx.close()
+
+
+def test_factored_and_read_basis_values(mp, oracle):
+    """The FP64 push forms each element's weighted basis values as (detJ/24) N_i(x_c)
+    when the mesh's values are exactly that (the brick generator's are), and reads
+    them otherwise.  With a time step short enough that no particle leaves its
+    element, the push's deposit of the new positions must reproduce the deposit of
+    the old ones (which always reads the values): bit for bit on the generated
+    mesh (factored path), and on a mesh with one element's values scaled by
+    1 + 2^-40 (read path; that element's nodes show the scaled values)."""
+    say("basis values: factored and read deposits")
+    om = oracle.Mesh(3, 3, 3, 6)
+    p = oracle.Particles(om.ne * 4).fill(om, 4, 0, DENSITY, temp=0.02, seed_offset=2)
+    interior = np.nonzero((om.boundary[om.elem_nodes] == 0).all(axis=1))[0]
+    k = int(next(e for e in p.elem[:p.n] if e in set(interior.tolist())))  # an element with particles, no wall node
+    rhs = {}
+    for name in ("factored", "read"):
+        wbv = om.wbv.copy()
+        if name == "read":
+            wbv[k] *= 1.0 + 2.0 ** -40
+        arrs = dict(nodes=om.nodes, elem_nodes=om.elem_nodes, neighbors=om.neighbors, normals=om.normals,
+                    jinv=om.jinv, detj=om.detj, grad=om.grad, wbv=wbv, boundary=om.boundary, rowptr=om.rowptr,
+                    col=om.col, val=om.val)
+        ctx = mp.Context(mp.desc_from_arrays(arrs), mp.default_config(sort_period=0))
+        upload(ctx, p)
+        first = ctx.time_step(1e-9)
+        r1 = ctx.rhs()
+        ctx.time_step(1e-9)
+        r2 = ctx.rhs()
+        assert first["crossings"] == 0, name
+        assert np.array_equal(r1, r2), name
+        rhs[name] = r1
+        ctx.close()
+    nodes_k = set(om.elem_nodes[k].tolist())
+    differ = set(np.nonzero(rhs["factored"] != rhs["read"])[0].tolist())
+    assert differ == nodes_k
