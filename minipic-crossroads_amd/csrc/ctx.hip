@@ -482,6 +482,20 @@ int mpic_create(const mpic_mesh_desc* d, const mpic_config* cfg, mpic_ctx** out)
     erec[e * 16 + 15] = meas;
   }
   c->m.wbv_factored = factored ? 1 : 0;
+  // Likewise the physical basis gradients: sum_j J^-1[j][d] kGradRef[i][j] from 0 in order j = 0..2 (PIC::PIC),
+  // formed by the FP64 push from the inverse Jacobian of its walk record when they are exactly that.
+  bool derived = ne > 0;
+  for (int64_t e = 0; e < ne && derived; ++e)
+    for (int i = 0; i < 4 && derived; ++i)
+      for (int dd = 0; dd < 3; ++dd) {
+        double g = 0;
+        for (int j = 0; j < 3; ++j) g += d->inv_jacobian[e * 9 + j * 3 + dd] * kGradRef[i][j];
+        if (std::memcmp(&g, d->grad_basis + e * 12 + i * 3 + dd, sizeof(double)) != 0) {
+          derived = false;
+          break;
+        }
+      }
+  c->m.grad_derived = derived ? 1 : 0;
   // Node -> element slot adjacency (slot = e*4 + k), element order.  The charge
   // of a periodic image node goes to its partner's row (PIC.cpp:132-134): the
   // partner gathers the image's slots, the image none.

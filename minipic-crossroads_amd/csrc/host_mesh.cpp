@@ -47,7 +47,7 @@ constexpr int kFaces[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
 // Face normal f = (n[C]-n[B]) x (n[A]-n[B]) (mesh.cpp:654-666).
 constexpr int kNa[4] = {0, 1, 0, 0}, kNb[4] = {1, 2, 3, 2}, kNc[4] = {3, 3, 2, 1};
 // P1 gradients on the reference tet (the cubature rule: p1_cubature.hpp).
-constexpr double kGradRef[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+using mpic::kGradRef;
 using mpic::kCub;
 using mpic::kCubW;
 

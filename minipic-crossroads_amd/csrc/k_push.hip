@@ -72,6 +72,33 @@ __device__ __forceinline__ void gather_E(const DevMesh& m, int e, const double R
       for (int d = 0; d < 3; ++d) E[d] += dof[i] * wt[j] * G[i * 3 + d];
 }
 
+// The same gather with the gradients formed from the element's inverse Jacobian
+// (DevMesh::grad_derived: bit for bit the mesh's gradients).
+__device__ __forceinline__ void gather_E_from_jinv(const DevMesh& m, int e, const double* Ji, const double R[3],
+                                                   double E[3]) {
+  const double wt[4] = {1 - R[0] - R[1] - R[2], R[0], R[1], R[2]};
+  const double2* fp = reinterpret_cast<const double2*>(m.ephi + (size_t)e * 4);
+  const double2 f01 = fp[0], f23 = fp[1];
+  const double dof[4] = {f01.x, f01.y, f23.x, f23.y};
+  double G[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double g = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g += Ji[j * 3 + d] * kGradRef[i][j];
+      G[i * 3 + d] = g;
+    }
+  E[0] = E[1] = E[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) E[d] += dof[i] * wt[j] * G[i * 3 + d];
+}
+
 // Boris push with B = 0, particle_move.hpp:364-392; v_ref = J^-1 v.
 __device__ __forceinline__ void kick(const double* Ji, double V[3], const double E[3], double qm, double dt,
                                      double scale, double VR[3]) {
@@ -394,7 +421,10 @@ __global__ __launch_bounds__(BLOCK, 3) void k_push_fused(KArgs a) {
         w = rc.w;
         g = load_erec(a.m.erec, e);
         double E[3];
-        gather_E(a.m, e, R, E);
+        if (a.m.grad_derived)
+          gather_E_from_jinv(a.m, e, g.J, R, E);
+        else
+          gather_E(a.m, e, R, E);
         double VR[3];
         kick(g.J, V, E, sp[32 + t], a.dt, a.scale, VR);
         const bool alive = walk(a.m, e, g, X, R, V, VR, a.dt, a.kill, wc);

@@ -71,6 +71,9 @@ struct DevMesh {
   // 1 when every weighted basis value is wbv[e][i][c] = (detJ_e * w) * N_i(x_c) bit for bit (the rule of
   // p1_cubature.hpp); erec[e][15] then holds detJ_e * w, and the FP64 push forms the 16 values itself
   int wbv_factored;
+  // 1 when every gradient is grad[e][i][d] = sum_j J^-1[j][d] kGradRef[i][j] bit for bit (p1_cubature.hpp):
+  // the FP64 push then forms them from the inverse Jacobian of its walk record
+  int grad_derived;
 };
 
 // Particles on the device are records (one 16-B aligned record per particle,

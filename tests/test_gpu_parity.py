@@ -289,3 +289,44 @@ def test_factored_and_read_basis_values(mp, oracle):
     nodes_k = set(om.elem_nodes[k].tolist())
     differ = set(np.nonzero(rhs["factored"] != rhs["read"])[0].tolist())
     assert differ == nodes_k
+
+
+def test_derived_and_read_gradients(mp, oracle):
+    """The FP64 push forms the element's physical basis gradients from its inverse
+    Jacobian when the mesh's gradients are exactly sum_j J^-1[j][d] G_ref[i][j] (the
+    generated mesh's are), and reads them otherwise.  Perturbing the gradients of
+    an element without particles selects the read path and must leave one time step
+    bit-identical to the derived path (and to the oracle); perturbing those of an
+    element with particles must change exactly their velocities."""
+    say("gradients: derived and read gathers")
+    om = oracle.Mesh(3, 3, 3, 6)
+    full = oracle.Particles(om.ne * 4).fill(om, 4, 0, DENSITY, temp=0.02, seed_offset=4)
+    k_empty, k_busy = 80, 78
+    keep = np.nonzero(full.elem[:full.n] != k_empty)[0]
+    p = oracle.Particles(len(keep))
+    for name in ("x", "xr", "v"):
+        getattr(p, name)[:, :len(keep)] = getattr(full, name)[:, keep]
+    for name in ("elem", "w", "type"):
+        getattr(p, name)[:len(keep)] = getattr(full, name)[keep]
+    p.id[:len(keep)] = np.arange(len(keep), dtype=p.id.dtype)
+    p.n = len(keep)
+    runs = {}
+    for name, k in (("derived", None), ("read_empty", k_empty), ("read_busy", k_busy)):
+        grad = om.grad.copy()
+        if k is not None:
+            grad[k] *= 1.0 + 2.0 ** -40
+        arrs = dict(nodes=om.nodes, elem_nodes=om.elem_nodes, neighbors=om.neighbors, normals=om.normals,
+                    jinv=om.jinv, detj=om.detj, grad=grad, wbv=om.wbv, boundary=om.boundary, rowptr=om.rowptr,
+                    col=om.col, val=om.val)
+        ctx = mp.Context(mp.desc_from_arrays(arrs), mp.default_config(sort_period=0))
+        upload(ctx, p)
+        ctx.time_step(0.05)
+        runs[name] = by_id(ctx.download())
+        ctx.close()
+    q = p.copy()
+    oracle.step(om, q, np.zeros(om.nn), 0.05)
+    for key in ("x", "v", "elem"):
+        assert np.array_equal(runs["derived"][key], runs["read_empty"][key]), key
+    assert np.array_equal(runs["derived"]["v"], q.v[:, :q.n])
+    changed = np.nonzero((runs["derived"]["v"] != runs["read_busy"]["v"]).any(axis=0))[0]
+    assert len(changed) > 0 and np.all(p.elem[changed] == k_busy)
