@@ -603,7 +603,7 @@ int mpic_batch_tune(mpic_batch* b, double dt) {
   // push: workgroups (particles per workgroup), on the field of the current
   // potential, into scratch deposit and counts; the input set is not modified
   BCHK(b, launch_field32(c->m, c->phi, c->m32.E, s));
-  const int cand_g[4] = {2048, 4096, 8192, 16384};
+  const int cand_g[4] = {4096, 8192, 18432, 32768};  // 512-thread workgroups
   best = 1e30f;
   for (int k = 0; k < 4; ++k) {
     BCHK(b, launch_zero(b->tune_acc, (size_t)c->m.ne * 8, s));

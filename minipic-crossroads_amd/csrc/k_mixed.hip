@@ -23,6 +23,10 @@ namespace {
 // The FP32 position carries ~3e-8 absolute error, i.e. a few 1e-6 in reference
 // coordinates of a 1/64 tet, so the band is set well above that.
 constexpr float LOC_EPS = 2e-4f;
+// Threads per k_push32 workgroup: eight wavefronts take adjacent batches of one range, so
+// the element records their walks load are shared in the CU's L1 (per launch at 100M
+// particles: 256 threads 4.09 ms, 384 4.49, 512 4.00, 768 4.15, 1024 5.13).
+constexpr int PUSH_BLOCK = 512;
 constexpr int LOC_CAP = 16;        // element moves of one point location
 constexpr double OWNER_TOL = 1e-9; // ownership check: tolerated negative coordinate
 
@@ -226,7 +230,7 @@ struct DepTable64 {
 
 template <int N>
 __device__ __forceinline__ void dep64_clear(DepTable64<N>& t) {
-  for (int k = threadIdx.x; k < N; k += BLOCK) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
     t.key[k] = DEP_EMPTY;
     t.cnt[k] = 0;
     t.val[k] = 0;
@@ -262,7 +266,7 @@ template <int N>
 __device__ __forceinline__ void dep64_flush(DepTable64<N>& t, unsigned long long* __restrict__ acc,
                                             int* __restrict__ count, const int32_t* __restrict__ ckey) {
   __syncthreads();
-  for (int k = threadIdx.x; k < N; k += BLOCK) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
     const int key = t.key[k];
     if (key != DEP_EMPTY) {
       const unsigned long long v = t.val[k];
@@ -402,10 +406,10 @@ __global__ __launch_bounds__(BLOCK) void k_field32(DevMesh m, const double* __re
   E[e] = make_float4((float)E0, (float)E1, (float)E2, 0.0f);
 }
 
-__global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
+__global__ __launch_bounds__(PUSH_BLOCK, 6) void k_push32(PushArgs32 a) {
   __shared__ DepTable64<DEP64_PUSH> table;
-  __shared__ float4 recx[BLOCK / 64][REC_PIECES];
-  __shared__ int recsrc[BLOCK / 64][64];
+  __shared__ float4 recx[PUSH_BLOCK / 64][REC_PIECES];
+  __shared__ int recsrc[PUSH_BLOCK / 64][64];
   __shared__ RareCount rare;
   __shared__ double sp[48];
   float4* sl = recx[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
@@ -418,10 +422,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
   fx128 ke{0, 0};
   unsigned kmax_hi = 0;  // high word of the largest KE term (terms are >= 0)
   int64_t lo, hi;
-  block_range(a.n, &lo, &hi);
+  block_range<PUSH_BLOCK>(a.n, &lo, &hi);
   __syncthreads();
   int chunk = 0;
-  for (int64_t base = lo; base < hi; base += BLOCK, ++chunk) {
+  for (int64_t base = lo; base < hi; base += PUSH_BLOCK, ++chunk) {
     const int64_t p = base + threadIdx.x;  // output position
     const int64_t s0 = base + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64;  // the wavefront's first
     const int64_t rem = hi - s0;
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_push32(PushArgs32 a) {
   }
   // an upper bound of the largest term: its high word with the low word all ones
   const unsigned long long kmax = kmax_hi ? ((unsigned long long)kmax_hi << 32) | 0xffffffffull : 0ull;
-  block_partial(a.partial, ke, kmax, wc, killed);
+  block_partial<PUSH_BLOCK>(a.partial, ke, kmax, wc, killed);
 }
 
 // Deposit of the current positions (weight_charge, PIC.cpp:139-160).
@@ -728,12 +732,12 @@ hipError_t launch_field32(const DevMesh& m, const double* phi, float4* E, hipStr
   hipLaunchKernelGGL(k_field32, dim3(grid_for(m.ne)), dim3(BLOCK), 0, s, m, phi, E);
   return hipGetLastError();
 }
-// Default grid of k_push32: ten rounds of the workgroups resident at once (six
-// per CU at 80 VGPRs: 15360 on 256 CUs), capped by PUSH32_MAX_GRID and the
-// number of 256-particle chunks.  Measured at 100M particles, time per step:
-// 6144 workgroups 9.19 ms, 7680 9.12, 8192 9.09, 9216 9.23, 12288 9.03, 15360
-// 9.02 (smaller ranges balance the walk's uneven cost better).
-constexpr int PUSH32_ROUNDS = 10;
+// Default grid of k_push32: 24 rounds of the workgroups resident at once (three
+// 512-thread workgroups per CU at 80 VGPRs: 18,432 on 256 CUs), capped by
+// PUSH32_MAX_GRID and the number of 512-particle batches.  Measured per launch at
+// 100M particles: 6 rounds 4.16 ms, 10 3.98, 16 3.93, 24 3.90, 32 3.91, 48 3.94
+// (smaller ranges balance the walk's uneven cost better).
+constexpr int PUSH32_ROUNDS = 24;
 int push32_grid(int64_t n) {
   static int cached[16];
   int dev = 0;
@@ -741,7 +745,7 @@ int push32_grid(int64_t n) {
   if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 16) {
     if (cached[dev] == 0) {
       int per_cu = 0, cus = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push32, BLOCK, 0) == hipSuccess &&
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_push32, PUSH_BLOCK, 0) == hipSuccess &&
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
           cus > 0)
         cached[dev] = std::min(PUSH32_MAX_GRID, PUSH32_ROUNDS * per_cu * cus);
@@ -750,7 +754,7 @@ int push32_grid(int64_t n) {
     }
     g = cached[dev];
   }
-  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+  const int64_t chunks = (n + PUSH_BLOCK - 1) / PUSH_BLOCK;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, chunks));
 }
 
@@ -761,7 +765,7 @@ hipError_t launch_push32(const PushArgs32& a, hipStream_t s) {
     return hipErrorInvalidValue;
   const unsigned g =
       a.grid > 0 ? (unsigned)(a.grid < PUSH32_MAX_GRID ? a.grid : PUSH32_MAX_GRID) : (unsigned)push32_grid(a.n);
-  hipLaunchKernelGGL(k_push32, dim3(g), dim3(BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_push32, dim3(g), dim3(PUSH_BLOCK), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_deposit32(const DevMesh& m, const DevParticles32& P, int64_t n, const double* species, int S,

@@ -357,10 +357,12 @@ struct WalkCount {
   int crossings, capped, nonpos, reflections, relocated;
 };
 
-// Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x].
+// Workgroup reduction of the push counters and the KE sum into partial[blockIdx.x]
+// (NT threads per workgroup).
+template <int NT = BLOCK>
 __device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsigned long long kmax, const WalkCount& wc,
                                               long long killed) {
-  __shared__ PushPartial sh[BLOCK / 64];
+  __shared__ PushPartial sh[NT / 64];
   unsigned long long klo = ke.lo;
   long long khi = ke.hi;
   long long c0 = wc.crossings, c1 = killed, c2 = wc.capped, c3 = wc.nonpos, c4 = wc.reflections, c5 = wc.relocated;
@@ -385,7 +387,7 @@ __device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsign
   __syncthreads();
   if (threadIdx.x == 0) {
     PushPartial r = sh[0];
-    for (int w = 1; w < BLOCK / 64; ++w) {
+    for (int w = 1; w < NT / 64; ++w) {
       fx128 t = fx_add(fx128{r.ke_lo, r.ke_hi}, fx128{sh[w].ke_lo, sh[w].ke_hi});
       r.ke_lo = t.lo;
       r.ke_hi = t.hi;
@@ -406,11 +408,12 @@ __device__ __forceinline__ void block_partial(PushPartial* out, fx128 ke, unsign
 // positions are accumulated in an LDS table and flushed every FLUSH_CHUNKS.
 constexpr int FLUSH_CHUNKS = 4;
 
+template <int NT = BLOCK>
 __device__ __forceinline__ void block_range(int64_t n, int64_t* lo, int64_t* hi) {
-  const int64_t chunks = (n + BLOCK - 1) / BLOCK;
+  const int64_t chunks = (n + NT - 1) / NT;
   const int64_t per = (chunks + gridDim.x - 1) / gridDim.x;
-  *lo = (int64_t)blockIdx.x * per * BLOCK;
-  const int64_t h = *lo + per * BLOCK;
+  *lo = (int64_t)blockIdx.x * per * NT;
+  const int64_t h = *lo + per * NT;
   *hi = h < n ? h : n;
 }
 

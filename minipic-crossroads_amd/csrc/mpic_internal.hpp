@@ -18,7 +18,7 @@ constexpr int BLOCK = 256;         // threads per workgroup for particle and row
 constexpr int WALK_CAP = 1000;     // crossings per particle per step (the reference has no cap)
 constexpr int ELL_SLICE = 64;      // rows per ELL slice = one wavefront
 constexpr int MAX_PARTIALS = 8192; // workgroups of the grid-stride particle kernels (upper bound)
-constexpr int PUSH32_MAX_GRID = 16384;  // workgroups of k_push32's default grid (upper bound)
+constexpr int PUSH32_MAX_GRID = 32768;  // workgroups of k_push32's grid (upper bound)
 constexpr int PARTIAL_SLOTS = PUSH32_MAX_GRID > MAX_PARTIALS ? PUSH32_MAX_GRID : MAX_PARTIALS;
 constexpr int DEFAULT_GRID = 4096; // their default number of workgroups
 constexpr int IR_MAX_OUTER = 40;   // refinement passes of the precision-form solve
