@@ -269,9 +269,12 @@ __device__ __forceinline__ void dep64_flush(DepTable64<N>& t, unsigned long long
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
     const int key = t.key[k];
     if (key != DEP_EMPTY) {
+      // the bucket load goes before the charge atomic: a wavefront's vector
+      // memory operations complete in order, so a load after it would wait for it
+      const int bkt = count ? ckey[key] : 0;
       const unsigned long long v = t.val[k];
       if (v) __hip_atomic_fetch_add(acc + key, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (count) atomicAdd(count + ckey[key], t.cnt[k]);
+      if (count) atomicAdd(count + bkt, t.cnt[k]);
     }
   }
   __syncthreads();

@@ -440,8 +440,9 @@ struct DepTable {
   fx128 val[DEP_TABLE][4];
 };
 
+template <int NT = BLOCK>
 __device__ __forceinline__ void dep_table_clear(DepTable& t) {
-  for (int k = threadIdx.x; k < DEP_TABLE; k += BLOCK) {
+  for (int k = threadIdx.x; k < DEP_TABLE; k += NT) {
     t.key[k] = DEP_EMPTY;
     t.cnt[k] = 0;
 #pragma unroll
@@ -501,21 +502,39 @@ __device__ __forceinline__ void dep_table_add(DepTable& t, unsigned long long* _
 }
 
 // Flushes every used entry with coalesced non-returning atomics (charges; counts
-// when count != null) and clears the table.  Must be called by the whole workgroup.
+// when count != null) and clears the table.  Must be called by the whole workgroup
+// of NT threads.  The counts go first, each thread issuing all its bucket loads
+// before any atomic: a wavefront's vector memory operations complete in order, so
+// a load issued after the charge atomics would wait for their round trips (one
+// wait per 16 entries in the charge loop).
+template <int NT = BLOCK>
 __device__ __forceinline__ void dep_table_flush(DepTable& t, unsigned long long* __restrict__ acc,
                                                 int* __restrict__ count, const int32_t* __restrict__ ckey) {
   __syncthreads();
+  if (count) {
+    constexpr int PER = (DEP_TABLE + NT - 1) / NT;
+    int key[PER], bkt[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int k = threadIdx.x + u * NT;
+      key[u] = k < DEP_TABLE ? t.key[k] : DEP_EMPTY;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) bkt[u] = key[u] != DEP_EMPTY ? ckey[key[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (key[u] != DEP_EMPTY) atomicAdd(count + bkt[u], t.cnt[threadIdx.x + u * NT]);
+  }
   const int sub = threadIdx.x & 15;  // node = sub/4, digit = sub%4
-  for (int e = threadIdx.x >> 4; e < DEP_TABLE; e += BLOCK / 16) {
+  for (int e = threadIdx.x >> 4; e < DEP_TABLE; e += NT / 16) {
     const int key = t.key[e];
     if (key != DEP_EMPTY) {
       const unsigned long long w = fx_digit(t.val[e][sub >> 2], sub & 3);
       if (w) __hip_atomic_fetch_add(acc + (size_t)key * 16 + sub, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (count && sub == 0) atomicAdd(count + ckey[key], t.cnt[e]);
     }
   }
   __syncthreads();
-  dep_table_clear(t);
+  dep_table_clear<NT>(t);
   __syncthreads();
 }
 
