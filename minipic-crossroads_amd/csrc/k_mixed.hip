@@ -384,9 +384,10 @@ __device__ __forceinline__ void wave_store_recs(float4* __restrict__ sl, Rec32* 
   float4* out = reinterpret_cast<float4*>(Q + s0);
   // piece 64 k + lane is part (lane + k) mod 3 of its record; part 2 is the weight/id word
   const int part = lane % 3;
-  if (lane < n3 && (whole || part != 2)) out[lane] = sl[lane];
-  if (64 + lane < n3 && (whole || part != 1)) out[64 + lane] = sl[64 + lane];
-  if (128 + lane < n3 && (whole || part != 0)) out[128 + lane] = sl[128 + lane];
+  const float4 w0 = sl[lane], w1 = sl[64 + lane], w2 = sl[128 + lane];
+  if (lane < n3 && (whole || part != 2)) out[lane] = w0;
+  if (64 + lane < n3 && (whole || part != 1)) out[64 + lane] = w1;
+  if (128 + lane < n3 && (whole || part != 0)) out[128 + lane] = w2;
   wave_sync();
 }
 

@@ -348,9 +348,10 @@ __device__ __forceinline__ void wave_store_recs64(float4* __restrict__ sl, Rec64
         reinterpret_cast<int4*>(sl + 6 * rr)[5] = int4{r.elem, (int)r.id, (int)r.type, 0};
       }
       wave_sync();
-      if (b + lane < n6) out[b + lane] = sl[lane];
-      if (b + 64 + lane < n6) out[b + 64 + lane] = sl[64 + lane];
-      if (b + 128 + lane < n6) out[b + 128 + lane] = sl[128 + lane];
+      const float4 w0 = sl[lane], w1 = sl[64 + lane], w2 = sl[128 + lane];
+      if (b + lane < n6) out[b + lane] = w0;
+      if (b + 64 + lane < n6) out[b + 64 + lane] = w1;
+      if (b + 128 + lane < n6) out[b + 128 + lane] = w2;
       wave_sync();
     }
   }
@@ -441,15 +442,16 @@ __global__ __launch_bounds__(BLOCK, 3) void k_push_fused(KArgs a) {
         meas = g.meas;
       }
     }
+    // node charges from the walk record before the stores: a wavefront's vector
+    // memory operations complete in order, so a use of the record's last loads
+    // after the stores would wait for the stores as well
+    fx128 c[4];
+    if (enew >= 0 && a.m.wbv_factored) particle_charge_factored(sixodet, meas, w, sp[t], a.S_charge, c);
     // records out (a dead particle's as read: the ranking drops it), then the element mirror
     if (nv > 0) wave_store_recs64(sl, a.Q.r, s0, nv, rc);
     if (p < hi) a.Q.elem[p] = rc.elem < 0 ? -1 : rc.elem;
     if (enew >= 0) {
-      fx128 c[4];
-      if (a.m.wbv_factored)
-        particle_charge_factored(sixodet, meas, w, sp[t], a.S_charge, c);
-      else
-        particle_charge(a.m, enew, a.m.sixodet[enew], w, sp[t], a.S_charge, c);
+      if (!a.m.wbv_factored) particle_charge(a.m, enew, a.m.sixodet[enew], w, sp[t], a.S_charge, c);
       dep_table_add(table, a.acc, a.count, a.m.bucket, enew, c);
       const double term = ke_term(w, sp[16 + t], V);
       ke = fx_add(ke, to_fixed(term, a.S_ke));
