@@ -282,6 +282,15 @@ class Context:
         self._chk(lib().mpic_particles_reserve(self.h, cap))
 
     def upload(self, x, xr, v, elem, w, typ, ids=None, n=None):
+        # the library reads rows of a [3][stride] layout and plain vectors: any
+        # other memory order (a transposed or fancy-indexed view) is copied first
+        x, xr, v = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, xr, v))
+        if not (x.shape == xr.shape == v.shape and x.shape[0] == 3):
+            raise ValueError("x, xr and v must be [3][stride] arrays of one shape")
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        typ = np.ascontiguousarray(typ, dtype=np.int32)
+        ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
         stride = x.shape[1]
         n = stride if n is None else n
         s = SoaHost(_p(x), _p(xr), _p(v), _p(elem), _p(w), _p(typ), _p(ids), stride)

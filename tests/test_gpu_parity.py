@@ -174,6 +174,34 @@ def test_sort_counts_and_determinism(mp, oracle):
     ctx.close()
 
 
+def test_rank_of_a_shuffled_list(mp, oracle):
+    """The list uploaded in random order: a ranking workgroup's 2,048 particles
+    then hold far more keys (elements, or sort buckets in the time step's
+    reordering) than its 256-entry table, and most of them take a slot of their
+    own with one atomic.  Counts, order and every particle's state against the
+    oracle, by element and through three steps reordering every second step."""
+    say("rank of a shuffled list kuhn16")
+    om, pm, ctx, bg = make(mp, oracle, 16, background=DENSITY, sort_period=2)
+    p = oracle.Particles(om.ne * 16).fill(om, 16, 0, DENSITY, temp=0.05, seed_offset=1)
+    order = np.random.default_rng(7).permutation(p.n)
+    ctx.reserve(p.cap)
+    ctx.upload(p.x[:, order], p.xr[:, order], p.v[:, order], p.elem[order], p.w[order], p.type[order], p.id[order],
+               n=p.n)
+    start = ctx.sort(want_start=True)
+    assert np.array_equal(np.diff(start), np.bincount(p.elem[:p.n], minlength=om.ne))
+    g = ctx.download()
+    assert np.all(np.diff(g["elem"]) >= 0)
+    assert_particles_equal(g, p, "after sort")
+    ctx.upload(p.x[:, order], p.xr[:, order], p.v[:, order], p.elem[order], p.w[order], p.type[order], p.id[order],
+               n=p.n)
+    phi = np.zeros(om.nn)
+    for s in range(3):
+        ctx.time_step(0.1)
+        oracle.step(om, p, phi, 0.1, bg=bg, S=ctx.charge_scale())
+        assert_particles_equal(ctx.download(), p, "step %d" % s)
+    ctx.close()
+
+
 def oracle_particles(oracle, g, idx=None):
     """Oracle particle set over downloaded arrays (optionally a subset)."""
     P = oracle.Particles(1)
