@@ -132,8 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
 // RANK_PROBES slots (a list far from key order: after a fill, or sorted by a
 // different key) takes its own slot with one atomic on cursor[key].  Every
 // particle's elements are loaded before any key is mapped or ranked.  Per
-// ranking at 100M particles: 1,024 particles and a 2,048-entry table that never
-// overflows 0.75 ms; this 0.48 ms.
+// ranking at 100M particles, between reorderings: 1,024 particles and a
+// 2,048-entry table that never overflows 0.75 ms; this 0.38 ms (the first
+// ranking after a fill, whose workgroups hold more keys than the table, 1.9 ms).
 constexpr int RANK_ITEMS = 8;
 constexpr int RANK_BITS = 8;
 constexpr int RANK_TABLE = 1 << RANK_BITS;
@@ -190,14 +191,17 @@ __global__ __launch_bounds__(BLOCK) void k_rank(const int32_t* __restrict__ elem
     if (kk >= 0) tcnt[k] = atomicAdd(cursor + kk, tcnt[k]);  // the key's base slot
   }
   __syncthreads();
+  // every destination first (the atomics of the particles without an entry all in
+  // flight together), then the stores back to back: a wavefront's memory
+  // operations complete in order, so a store between them would be waited for
 #pragma unroll
-  for (int u = 0; u < RANK_ITEMS; ++u) {
-    const int32_t p = (int32_t)(p0 + u * BLOCK);
-    if (slot[u] >= 0)
-      perm[tcnt[slot[u]] + rank[u]] = p;
-    else if (slot[u] == -2)
-      perm[atomicAdd(cursor + key[u], 1)] = p;
-  }
+  for (int u = 0; u < RANK_ITEMS; ++u)
+    if (slot[u] == -2) rank[u] = atomicAdd(cursor + key[u], 1);
+#pragma unroll
+  for (int u = 0; u < RANK_ITEMS; ++u) rank[u] = slot[u] >= 0 ? tcnt[slot[u]] + rank[u] : slot[u] == -2 ? rank[u] : -1;
+#pragma unroll
+  for (int u = 0; u < RANK_ITEMS; ++u)
+    if (rank[u] >= 0) perm[rank[u]] = (int32_t)(p0 + u * BLOCK);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_permute(DevParticles src, DevParticles dst, const int32_t* __restrict__ perm,
