@@ -127,16 +127,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(const int32_t* __restrict_
 // the key's count to cursor[key] with one atomic, and every particle's slot is
 // the returned base plus its local rank.  Between reorderings a particle moves
 // only to neighbouring elements, so a workgroup's 2,048 particles hold a few
-// dozen keys: a small table (256 entries, cleared and scanned in one pass of
+// dozen keys: a small table (512 entries, cleared and scanned in two passes of
 // the workgroup) holds them, and a particle whose key finds no entry within
 // RANK_PROBES slots (a list far from key order: after a fill, or sorted by a
 // different key) takes its own slot with one atomic on cursor[key].  Every
 // particle's elements are loaded before any key is mapped or ranked.  Per
 // ranking at 100M particles, between reorderings: 1,024 particles and a
-// 2,048-entry table that never overflows 0.75 ms; this 0.38 ms (the first
-// ranking after a fill, whose workgroups hold more keys than the table, 1.9 ms).
+// 2,048-entry table that never overflows 0.75 ms; this 0.39 ms, and 0.84 ms for
+// the first ranking after a fill, whose workgroups hold a few hundred keys (a
+// 256-entry table: 0.38 and 1.9 ms).
 constexpr int RANK_ITEMS = 8;
-constexpr int RANK_BITS = 8;
+constexpr int RANK_BITS = 9;
 constexpr int RANK_TABLE = 1 << RANK_BITS;
 constexpr int RANK_PROBES = 32;
 static_assert(RANK_TABLE % BLOCK == 0 && RANK_PROBES <= RANK_TABLE, "rank table");

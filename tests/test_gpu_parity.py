@@ -177,7 +177,7 @@ def test_sort_counts_and_determinism(mp, oracle):
 def test_rank_of_a_shuffled_list(mp, oracle):
     """The list uploaded in random order: a ranking workgroup's 2,048 particles
     then hold far more keys (elements, or sort buckets in the time step's
-    reordering) than its 256-entry table, and most of them take a slot of their
+    reordering) than its 512-entry table, and most of them take a slot of their
     own with one atomic.  Counts, order and every particle's state against the
     oracle, by element and through three steps reordering every second step."""
     say("rank of a shuffled list kuhn16")
