@@ -144,20 +144,20 @@ int mg_solve(mpic_ctx* c, double tol, int max_iter, int* iters, double* relres) 
       restart = false;
     }
     batch = 2;
+    // the check of the true residual is queued behind the batch and runs only if
+    // the batch met the tolerance: one wait per batch
+    HIPCHK(c, launch_mg_recheck(c->m, c->m32, g, c->phi, c->rhs, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->mg_h, g.st, sizeof(MGState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, wait_stream(c->stream));
     const MGState s = *c->mg_h;
-    if (!s.done) {
+    if (!s.armed) {
+      if (s.done) break;  // iteration limit or breakdown (done, not converged)
       // the V-cycle after iteration max_iter - 1 stops the solve
       if (it >= max_iter) return fail(c, MPIC_ERR_STATE, "multigrid CG: no stop at the iteration limit");
       continue;
     }
-    if (!s.converged) break;  // iteration limit or breakdown
-    HIPCHK(c, launch_mg_recheck(c->m, c->m32, g, c->phi, c->rhs, s.iters, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->mg_h, g.st, sizeof(MGState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, wait_stream(c->stream));
-    if (c->mg_h->done || ++rechecks > MG_MAX_RECHECKS) break;
-    // continue from the true residual
+    if (s.done || ++rechecks > MG_MAX_RECHECKS) break;
+    // the true residual missed the tolerance: continue from it
     HIPCHK(c, launch_mg_vcycle(c->m, c->m32, g, c->phi, c->rhs, s.iters, c->stream));
     it = s.iters;
     restart = true;

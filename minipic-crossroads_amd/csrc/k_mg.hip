@@ -53,14 +53,15 @@ __device__ __forceinline__ double reduce(const double* __restrict__ part, int P)
   return block_sum(thread_partials(part, P));
 }
 
-// The stop flag, read once per workgroup (a launch whose workgroup 0 may set
-// it must not let the waves of another workgroup disagree before a barrier).
-__device__ __forceinline__ bool wg_done(const MGState* st) {
+// A state flag, read once per workgroup (a launch whose workgroup 0 may set it
+// must not let the waves of another workgroup disagree before a barrier).
+__device__ __forceinline__ bool wg_flag(const int* f) {
   __shared__ int flag;
-  if (threadIdx.x == 0) flag = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) flag = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   return flag != 0;
 }
+__device__ __forceinline__ bool wg_done(const MGState* st) { return wg_flag(&st->done); }
 
 // Level 0 points: every node, numbered as the brick generator does.
 __device__ __forceinline__ int nidx(const MGBuffers& g, int i, int j, int k) {
@@ -75,6 +76,7 @@ struct MGArgs {
   const double* b;     // rhs
   int it;
   int restart;
+  int cond;  // k_mg_init / k_mg_test as the check of a converged solve: only when armed
 };
 
 // ---- restriction, prolongation and the tiles of the coarse levels ----
@@ -155,28 +157,43 @@ __global__ void k_mg_reopen(MGState* st) {
   st->converged = 0;
 }
 
-// The stop test of iteration it on the partials of k_mg_init, in one workgroup
-// (the check of the true residual after the recursive one met the tolerance).
+// Opens the check of the true residual only when the recursive one has met the
+// tolerance; armed says whether it did (the check's kernels return at once if not).
+__global__ void k_mg_reopen_if(MGState* st) {
+  const int arm = st->done && st->converged ? 1 : 0;
+  st->armed = arm;
+  if (arm) {
+    st->done = 0;
+    st->converged = 0;
+  }
+}
+
+// The stop test on the partials of k_mg_init, in one workgroup (the check of the
+// true residual after the recursive one met the tolerance, at the iteration that
+// met it).
 __global__ __launch_bounds__(BLOCK) void k_mg_test(MGArgs a) {
   MGState* st = a.g.st;
+  if (a.cond && !wg_flag(&st->armed)) return;
+  const int it = st->iters;
   const double rr = reduce(a.g.part_rr, a.g.nchunks);
   const double bb = reduce(a.g.part_bb, a.g.nchunks);
   if (threadIdx.x == 0) {
     const bool conv = rr == 0.0 || rr < st->tol2 * bb;
     st->rr = rr;
     st->bb = bb;
-    st->iters = a.it;
-    if (conv || a.it >= st->max_iter) {
+    if (conv || it >= st->max_iter) {
       st->done = 1;
       st->converged = conv ? 1 : 0;
     }
   }
 }
 
-// r = b - A x (FP64), b0 = (float) r, x0 = omega D^-1 b0; partials of b.b and r.r.
+// r = b - A x (FP64), b0 = (float) r, x0 = omega D^-1 b0; partials of b.b and r.r
+// (as a check, a.cond: only when armed).
 __global__ __launch_bounds__(BLOCK) void k_mg_init(MGArgs a) {
   const int chunk = xcd_chunk(blockIdx.x, a.g.nchunks);
   if (chunk < 0) return;
+  if (a.cond && !wg_flag(&a.g.st->armed)) return;
   const int i = chunk * BLOCK + threadIdx.x;
   double tb = 0.0, tr = 0.0;
   if (i < a.m.nn) {
@@ -435,6 +452,7 @@ MGArgs make_args(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, c
   a.b = b;
   a.it = it;
   a.restart = restart ? 1 : 0;
+  a.cond = 0;
   return a;
 }
 
@@ -488,11 +506,12 @@ hipError_t launch_mg_iter(const DevMesh& m, const Mesh32& m32, MGBuffers& g, dou
   return hipGetLastError();
 }
 
-hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b,
                              hipStream_t s) {
   if (!valid(g)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mg_reopen, dim3(1), dim3(1), 0, s, g.st);
-  const MGArgs a = make_args(m, m32, g, x, b, it, false);
+  hipLaunchKernelGGL(k_mg_reopen_if, dim3(1), dim3(1), 0, s, g.st);
+  MGArgs a = make_args(m, m32, g, x, b, 0, false);
+  a.cond = 1;
   hipLaunchKernelGGL(k_mg_init, dim3(xcd_grid(g.nchunks)), dim3(BLOCK), 0, s, a);
   hipLaunchKernelGGL(k_mg_test, dim3(1), dim3(BLOCK), 0, s, a);
   return hipGetLastError();

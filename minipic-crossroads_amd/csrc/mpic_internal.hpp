@@ -417,6 +417,8 @@ struct MGLevelDev {
 struct MGState {
   int done, converged, iters, max_iter;
   double tol2, bb, rr;
+  int armed;  // the last recheck launch found the solve converged and tested the true residual
+  int pad;
 };
 struct MGBuffers {
   int nlev = 0;             // coarse levels 1..nlev; the last is solved by its dense inverse
@@ -441,9 +443,11 @@ hipError_t launch_mg_start(const DevMesh& m, const Mesh32& m32, MGBuffers& g, do
 // update, then the V-cycle of iteration it + 1 (which first tests convergence).
 hipError_t launch_mg_iter(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
                           bool restart, hipStream_t s);
-// After a stop: reopens the solve, recomputes r = b - A x in FP64 and tests it
-// (the state is done again if it meets the tolerance or the limit).
-hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b, int it,
+// Queued after every batch of iterations: when the recursive residual has met the
+// tolerance (done and converged), reopens the solve, recomputes r = b - A x in
+// FP64 and tests it (done again if it meets the tolerance or the limit) and sets
+// armed; otherwise clears armed and leaves the solve as it is.
+hipError_t launch_mg_recheck(const DevMesh& m, const Mesh32& m32, MGBuffers& g, double* x, const double* b,
                              hipStream_t s);
 // The V-cycle of iteration it on the current residual (after a failed recheck:
 // the iteration then restarts from it).
